@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Benchmark of the plain multisplit hot path on one MI355X.
+
+Default workload: stable key-value multisplit of 2^27 uint32 keys (uniform over
+[0, 2^32)) with uint32 values (value = original index) into 256 buckets by the
+range identifier floor(key * 256 / 2^32).  A step is one full multisplit call
+(prescan -> scan -> postscan) with inputs already resident in HBM.
+
+Prints ONE JSON line: whole-job throughput in Gkeys/s, the postscan kernel's
+achieved HBM bandwidth against the 8 TB/s peak (algorithmic bytes: 16 B/key,
+key+value read and written once; the whole call moves 20 B/key), and the CPU
+oracle timed on the host as a reported baseline.
+
+Options other than the driver's (--gpus/--steps/--warmup) are for
+measurement runs: --workload picks another configuration, --no-cpu skips the
+CPU baseline, --no-stage-events times without per-stage HIP events.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
+    p.add_argument("--workload", default="ms_kv_256",
+                   help="ms_{kv,ko}_<B>[_<dist>][_n<log2n>] | sort{7,8}_{kv,ko} | "
+                        "hist256 | compact{50,5} | copy")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-stage-events", action="store_true")
+    p.add_argument("--cpu-reps", type=int, default=5)
+    return p.parse_args()
+
+
+def traffic_from_profiles(workload):
+    """HBM bytes per postscan launch from committed rocprofv3 PMC summaries."""
+    path = os.path.join(PROFILES, f"traffic_{workload}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def parse_workload(w):
+    parts = w.split("_")
+    cfg = {"kind": parts[0]}
+    if parts[0] == "ms":
+        cfg["kv"] = parts[1] == "kv"
+        cfg["B"] = int(parts[2])
+        cfg["dist"] = "uniform"
+        cfg["log2n"] = 27
+        for extra in parts[3:]:
+            if extra.startswith("n"):
+                cfg["log2n"] = int(extra[1:])
+            else:
+                cfg["dist"] = extra
+    return cfg
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the multi-GPU sharding design is described in DESIGN.md); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import gpumultisplit_amd as gms
+    from gpumultisplit_amd import workloads as W
+
+    gms.load_library()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    cfg = parse_workload(args.workload)
+    if cfg["kind"] != "ms":
+        raise SystemExit(f"workload {args.workload!r} not available in this bench version")
+
+    n = 1 << cfg["log2n"]
+    B = cfg["B"]
+    kv = cfg["kv"]
+    print(f"[bench] workload={args.workload} n={n} B={B} kv={kv} dist={cfg['dist']}",
+          file=sys.stderr, flush=True)
+    if cfg["dist"] == "uniform":
+        g = torch.Generator(device=dev)
+        g.manual_seed(12345)
+        kin = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=dev, generator=g)
+    else:
+        kin = torch.from_numpy(W.keys_for(cfg["dist"], n, B, seed=12345).view(np.int32)).to(dev)
+    vin = torch.arange(n, dtype=torch.int32, device=dev) if kv else None
+    kout = torch.empty_like(kin)
+    vout = torch.empty_like(kin) if kv else None
+    ctx = gms.multisplit_context(B)
+    (gms.multisplit_allocate_key_value if kv else gms.multisplit_allocate_key_only)(n, ctx)
+    f = gms.range_bucket(B)
+    desc = f.desc()
+    L = gms.lib()
+    h, kp, vp = ctx.handle, kin.data_ptr(), (vin.data_ptr() if kv else None)
+    kop, vop = kout.data_ptr(), (vout.data_ptr() if kv else None)
+
+    def step():
+        st = L.gms_multisplit(h, kp, vp, kop, vop, n, desc, None, stream)
+        if st != 0:
+            raise gms.GmsError(st, "gms_multisplit")
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    stage_events = not args.no_stage_events
+    gms.set_profiling(ctx, stage_events)
+    gms.read_profile(ctx)  # reset
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    pre_ms, scan_ms, post_ms, calls = gms.read_profile(ctx) if stage_events else (0, 0, 0, 0)
+    gms.set_profiling(ctx, False)
+    print(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step", file=sys.stderr, flush=True)
+
+    gkeys = n / (ms_per_step * 1e-3) / 1e9
+    bpk_call = 20 if kv else 12
+    bpk_post = 16 if kv else 8
+    roofline = None
+    stages = None
+    if stage_events and calls:
+        post_avg = post_ms / calls
+        achieved = n * bpk_post / (post_avg * 1e-3) / 1e9
+        tr = traffic_from_profiles(args.workload)
+        roofline = {"kernel": "ms_postscan_kernel", "bound": "hbm", "achieved": round(achieved, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                    "traffic": tr, "algorithmic_bytes": n * bpk_post,
+                    "avg_launch_us": round(post_avg * 1e3, 2)}
+        stages = {"prescan_us": round(pre_ms / calls * 1e3, 2), "scan_us": round(scan_ms / calls * 1e3, 2),
+                  "postscan_us": round(post_avg * 1e3, 2)}
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        print("[bench] cpu baseline (oracle, 1 thread)...", file=sys.stderr, flush=True)
+        keys = W.uniform_keys(n, seed=12345)
+        vals = np.arange(n, dtype=np.uint32) if kv else None
+        ts = []
+        for _ in range(args.cpu_reps):
+            c0 = time.perf_counter()
+            oracle.multisplit(keys, B, oracle.RANGE, 0, vals)
+            ts.append(time.perf_counter() - c0)
+        med = statistics.median(ts)
+        cpu = {"value": round(n / med / 1e9, 4), "unit": "Gkeys/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_reps} runs (median) of the full workload ({n} keys) with the "
+                         f"single-threaded C oracle on the host, {os.cpu_count()} host threads present"}
+
+    out = {
+        "metric": f"{'key-value' if kv else 'key-only'} multisplit throughput, {n} uint32 keys, "
+                  f"{B} buckets (range identifier)",
+        "value": round(gkeys, 3),
+        "unit": "Gkeys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": f"synthetic ({cfg['dist']} keys, values = original index)",
+        "config": {"workload": args.workload, "n": n, "buckets": B, "key_value": kv,
+                   "bucket_identifier": "range", "tile_keys": gms.lib().gms_subproblem_size(B, int(kv))},
+        "call_hbm_fraction": round(gkeys * bpk_call / HBM_PEAK_GBS, 4),
+        "roofline": roofline,
+        "stages": stages,
+        "cpu_baseline": cpu,
+    }
+    gms.multisplit_release_memory(ctx)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,547 @@
+"""gpumultisplit_amd -- MI355X (gfx950) multisplit, radix sort, histogram and
+compaction behind the reference's API names.
+
+The compute path is ``libgpumultisplit.so`` (hand-written HIP kernels, see
+``csrc/``), reached through its C ABI (``include/gpumultisplit.h``) with
+ctypes.  This module mirrors the reference's C++ host API
+(src/api/multisplit.cuh, src/api/multisplit_sort.cuh,
+src/api/multisplit_sort_pairs.cuh,
+src/kernels/compaction/multisplit2_compaction.cuh) with the same names and
+argument order.  Device buffers are torch tensors on the GPU (any 32-bit
+dtype; the bits are moved) or raw integer device pointers.
+
+Differences from the reference, all deliberate:
+* errors raise ``GmsError`` instead of printing or silently doing nothing;
+* one allocation serves every call with n <= the allocated n;
+* ``multisplit_offset`` (host array of B bucket starts) is filled with one
+  copy instead of B-1 blocking copies.
+
+There is no CPU fallback: when the library is missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+from . import workloads  # noqa: F401  (seeded synthetic inputs)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpumultisplit.so")
+
+GMS_SUCCESS = 0
+STATUS_NAMES = {
+    0: "GMS_SUCCESS", 1: "GMS_ERR_INVALID_ARG", 2: "GMS_ERR_UNSUPPORTED_BUCKETS",
+    3: "GMS_ERR_CAPACITY", 4: "GMS_ERR_NOT_ALLOCATED", 5: "GMS_ERR_ALLOC", 6: "GMS_ERR_HIP",
+    7: "GMS_ERR_ALIAS", 8: "GMS_ERR_NO_DEVICE",
+}
+
+BUCKET_RANGE, BUCKET_DELTA, BUCKET_DIGIT, BUCKET_IDENTITY, BUCKET_LOWBITS = 0, 1, 2, 3, 4
+
+
+class GmsError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        msg = STATUS_NAMES.get(status, str(status))
+        if _lib is not None:
+            msg += ": " + _lib.gms_status_string(status).decode()
+        super().__init__(f"{what}: {msg}" if what else msg)
+
+
+class gms_bucket_desc(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("param", ctypes.c_uint32), ("width", ctypes.c_uint32)]
+
+
+_lib = None
+
+# (name, restype, argtypes) of every exported symbol, in header order.
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_I = ctypes.c_int
+_PP = ctypes.POINTER(ctypes.c_void_p)
+_SIGS = [
+    ("gms_status_string", ctypes.c_char_p, [_I]),
+    ("gms_abi_version", _U32, []),
+    ("gms_bucket_ids_host", _I, [_U32, gms_bucket_desc, _P, _P, ctypes.c_size_t]),
+    ("gms_multisplit_create", _I, [_U32, _PP]),
+    ("gms_multisplit_allocate", _I, [_P, _U32, _I]),
+    ("gms_multisplit_release", _I, [_P]),
+    ("gms_multisplit_destroy", _I, [_P]),
+    ("gms_multisplit_key_only", _I, [_P, _P, _P, _U32, gms_bucket_desc, _P, _P]),
+    ("gms_multisplit_key_value", _I, [_P, _P, _P, _P, _P, _U32, gms_bucket_desc, _P, _P]),
+    ("gms_multisplit", _I, [_P, _P, _P, _P, _P, _U32, gms_bucket_desc, _P, _P]),
+    ("gms_subproblem_size", _U32, [_U32, _I]),
+    ("gms_multisplit_set_profiling", _I, [_P, _I]),
+    ("gms_multisplit_profile", _I, [_P, ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_U32)]),
+    ("gms_multisplit_prescan_matrix", _I, [_P, _P, _U32, gms_bucket_desc, _I, _P, _P]),
+    ("gms_exclusive_scan_u32", _I, [_P, _P, _P, _U32, _P]),
+    ("gms_sort_create", _I, [_PP]),
+    ("gms_sort_allocate", _I, [_P, _U32, _U32, _I]),
+    ("gms_sort_release", _I, [_P]),
+    ("gms_sort_destroy", _I, [_P]),
+    ("gms_sort", _I, [_P, _P, _P, _P, _P, _U32, _U32, _P]),
+    ("gms_compaction_create", _I, [_PP]),
+    ("gms_compaction_allocate", _I, [_P, _U32, _I]),
+    ("gms_compaction_release", _I, [_P]),
+    ("gms_compaction_destroy", _I, [_P]),
+    ("gms_compaction", _I, [_P, _P, _P, _P, _P, _U32, gms_bucket_desc, _P, _P, _P]),
+    ("gms_histogram_u32", _I, [_P, _U32, _P, _U32, gms_bucket_desc, _P]),
+    ("gms_histogram_even_f32", _I, [_P, _U32, _P, _U32, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, _P]),
+    ("gms_copy_u32", _I, [_P, _P, ctypes.c_size_t, _P]),
+]
+EXPORTED_SYMBOLS = [s[0] for s in _SIGS]
+
+
+def build(verbose: bool = False) -> None:
+    """Compile every HIP source for gfx950 into libgpumultisplit.so (in-tree)."""
+    jobs = str(min(16, os.cpu_count() or 8))
+    out = None if verbose else subprocess.DEVNULL
+    subprocess.run(["make", "-j", jobs, "-C", os.path.join(_HERE, "csrc")], check=True, stdout=out)
+
+
+def load_library():
+    """Load libgpumultisplit.so.  Raises if it is missing: there is no fallback."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing; run gpumultisplit_amd.build() "
+                               "(or __graft_entry__.build()) first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def lib():
+    return load_library()
+
+
+def _check(status: int, what: str) -> None:
+    if status != GMS_SUCCESS:
+        raise GmsError(status, what)
+
+
+def _ptr(x):
+    """Device pointer of a torch tensor (or an int address); None stays None."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        if hasattr(x, "is_cuda") and not x.is_cuda:
+            raise ValueError("device buffer expected, got a host tensor")
+        if x.element_size() != 4:
+            raise ValueError("32-bit elements expected")
+        return x.data_ptr()
+    raise TypeError(f"cannot take a device pointer of {type(x)!r}")
+
+
+def _stream(*tensors):
+    for t in tensors:
+        if t is not None and hasattr(t, "is_cuda") and t.is_cuda:
+            import torch
+            return torch.cuda.current_stream(t.device).cuda_stream
+    return None
+
+
+def _check_len(name, t, n):
+    if t is not None and hasattr(t, "numel") and t.numel() < n:
+        raise ValueError(f"{name} holds {t.numel()} elements, fewer than num_elements={n}")
+
+
+# ---------------------------------------------------------------------------
+# Bucket identifiers.  Same names as the reference's functors; each maps to a
+# C-ABI descriptor and also evaluates on the host (for checks).
+# ---------------------------------------------------------------------------
+class _Bucket:
+    kind = BUCKET_RANGE
+    param = 0
+    width = 0
+
+    def desc(self) -> gms_bucket_desc:
+        return gms_bucket_desc(self.kind, self.param, self.width)
+
+
+class range_bucket(_Bucket):
+    """floor(key * B / 2^32) for B buckets (the sweep's range identifier)."""
+    kind = BUCKET_RANGE
+
+    def __init__(self, num_buckets: int):
+        self.num_buckets = num_buckets
+
+    def __call__(self, a: int) -> int:
+        return (int(a) * self.num_buckets) >> 32
+
+
+class delta_func(_Bucket):
+    """key / delta (src/main/main_multisplit.cu:41-47)."""
+    kind = BUCKET_DELTA
+
+    def __init__(self, delta: int):
+        if delta < 1:
+            raise ValueError("delta must be >= 1")
+        self.delta_ = self.param = int(delta)
+
+    def __call__(self, a: int) -> int:
+        return int(a) // self.delta_
+
+
+class identity_bucket(_Bucket):
+    """key itself (src/main/main_multisplit.cu:34-39)."""
+    kind = BUCKET_IDENTITY
+
+    def __call__(self, a: int) -> int:
+        return int(a)
+
+
+class test_bucket(_Bucket):
+    """key & ((1 << flag) - 1) (src/main/main_compaction.cu:27-32)."""
+    kind = BUCKET_LOWBITS
+
+    def __init__(self, flag: int):
+        self.flag = self.param = int(flag)
+
+    def __call__(self, a: int) -> int:
+        return int(a) & ((1 << self.flag) - 1)
+
+
+def _digit_functor(bits: int):
+    class _Digit(_Bucket):
+        kind = BUCKET_DIGIT
+        width = bits
+
+        def __init__(self, x: int):
+            self.x = self.param = int(x)
+
+        def __call__(self, a: int) -> int:
+            return (int(a) >> self.x) & ((1 << bits) - 1)
+
+    _Digit.__name__ = f"ms_bucket_sort_{bits}bit"
+    _Digit.__doc__ = f"(key >> x) & {(1 << bits) - 1} (src/api/multisplit_sort.cuh:63-98)."
+    return _Digit
+
+
+ms_bucket_sort_2bit = _digit_functor(2)
+ms_bucket_sort_4bit = _digit_functor(4)
+ms_bucket_sort_5bit = _digit_functor(5)
+ms_bucket_sort_6bit = _digit_functor(6)
+ms_bucket_sort_7bit = _digit_functor(7)
+ms_bucket_sort_8bit = _digit_functor(8)
+
+
+def bucket_ids_host(keys, num_buckets: int, bucket_identifier):
+    """Bucket ids computed by the library's own host copy of the kernel code."""
+    import numpy as np
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    ids = np.empty_like(k)
+    _check(lib().gms_bucket_ids_host(num_buckets, bucket_identifier.desc(), k.ctypes.data,
+                                     ids.ctypes.data, k.size), "gms_bucket_ids_host")
+    return ids
+
+
+# ---------------------------------------------------------------------------
+# Multisplit (src/api/multisplit.cuh)
+# ---------------------------------------------------------------------------
+def subproblem_size_bms_key_only(num_buckets: int) -> int:
+    return int(lib().gms_subproblem_size(num_buckets, 0))
+
+
+def subproblem_size_bms_key_value(num_buckets: int) -> int:
+    return int(lib().gms_subproblem_size(num_buckets, 1))
+
+
+class multisplit_context:
+    """multisplit_context(n_buckets) (src/api/multisplit.cuh:29-47)."""
+
+    def __init__(self, n_buckets: int):
+        self.num_buckets = int(n_buckets)
+        self.size_sub_prob = 0
+        self.num_sub_prob = 0
+        self.num_blocks = 0
+        self.capacity = 0
+        h = ctypes.c_void_p()
+        _check(lib().gms_multisplit_create(self.num_buckets, ctypes.byref(h)), "multisplit_context")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.gms_multisplit_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def _ms_allocate(num_elements: int, context: multisplit_context, kv: int) -> None:
+    _check(lib().gms_multisplit_allocate(context.handle, int(num_elements), kv),
+           "multisplit_allocate")
+    context.capacity = int(num_elements)
+    context.size_sub_prob = int(lib().gms_subproblem_size(context.num_buckets, kv))
+    context.num_sub_prob = context.num_blocks = (
+        (int(num_elements) + context.size_sub_prob - 1) // context.size_sub_prob)
+
+
+def multisplit_allocate_key_only(num_elements: int, context: multisplit_context) -> None:
+    """src/api/multisplit.cuh:50-61."""
+    _ms_allocate(num_elements, context, 0)
+
+
+def multisplit_allocate_key_value(num_elements: int, context: multisplit_context) -> None:
+    """src/api/multisplit.cuh:63-74."""
+    _ms_allocate(num_elements, context, 1)
+
+
+def multisplit_release_memory(context: multisplit_context) -> None:
+    """src/api/multisplit.cuh:76-79."""
+    _check(lib().gms_multisplit_release(context.handle), "multisplit_release_memory")
+
+
+def _offsets_buffer(multisplit_offset, num_buckets):
+    if multisplit_offset is None:
+        return None, None
+    import numpy as np
+    if isinstance(multisplit_offset, np.ndarray):
+        if multisplit_offset.dtype != np.uint32 or not multisplit_offset.flags.c_contiguous:
+            raise ValueError("multisplit_offset must be a contiguous uint32 array")
+        if multisplit_offset.size < num_buckets:
+            raise ValueError("multisplit_offset needs num_buckets entries")
+        return multisplit_offset.ctypes.data, None
+    if hasattr(multisplit_offset, "data_ptr") and not multisplit_offset.is_cuda:
+        if multisplit_offset.numel() < num_buckets or multisplit_offset.element_size() != 4:
+            raise ValueError("multisplit_offset needs num_buckets 32-bit entries")
+        return multisplit_offset.data_ptr(), None
+    raise TypeError("multisplit_offset must be a host numpy array or CPU tensor")
+
+
+def multisplit_key_only(d_key_in, d_key_out, num_elements: int, context: multisplit_context,
+                        bucket_identifier, multisplit_offset=None) -> None:
+    """Stable key-only multisplit (src/api/multisplit.cuh:81-197)."""
+    _check_len("d_key_in", d_key_in, num_elements)
+    _check_len("d_key_out", d_key_out, num_elements)
+    h_off, _ = _offsets_buffer(multisplit_offset, context.num_buckets)
+    _check(lib().gms_multisplit_key_only(context.handle, _ptr(d_key_in), _ptr(d_key_out),
+                                         int(num_elements), bucket_identifier.desc(), h_off,
+                                         _stream(d_key_in, d_key_out)), "multisplit_key_only")
+
+
+def multisplit_key_value(d_key_in, d_value_in, d_key_out, d_value_out, num_elements: int,
+                         context: multisplit_context, bucket_identifier,
+                         multisplit_offset=None) -> None:
+    """Stable key-value multisplit (src/api/multisplit.cuh:199-313)."""
+    for name, t in (("d_key_in", d_key_in), ("d_value_in", d_value_in),
+                    ("d_key_out", d_key_out), ("d_value_out", d_value_out)):
+        _check_len(name, t, num_elements)
+    h_off, _ = _offsets_buffer(multisplit_offset, context.num_buckets)
+    _check(lib().gms_multisplit_key_value(context.handle, _ptr(d_key_in), _ptr(d_value_in),
+                                          _ptr(d_key_out), _ptr(d_value_out), int(num_elements),
+                                          bucket_identifier.desc(), h_off,
+                                          _stream(d_key_in, d_key_out)), "multisplit_key_value")
+
+
+def multisplit(d_key_in, d_key_out, num_elements: int, context: multisplit_context,
+               bucket_identifier, d_value_in=None, d_value_out=None, d_offsets=None,
+               stream=None) -> None:
+    """Asynchronous form: optional values, bucket starts into device memory."""
+    _check(lib().gms_multisplit(context.handle, _ptr(d_key_in), _ptr(d_value_in), _ptr(d_key_out),
+                                _ptr(d_value_out), int(num_elements), bucket_identifier.desc(),
+                                _ptr(d_offsets),
+                                stream if stream is not None else _stream(d_key_in, d_key_out)),
+           "gms_multisplit")
+
+
+def set_profiling(context: multisplit_context, enable: bool) -> None:
+    _check(lib().gms_multisplit_set_profiling(context.handle, int(bool(enable))), "set_profiling")
+
+
+def read_profile(context: multisplit_context):
+    """(ms_prescan, ms_scan, ms_postscan, calls) summed since the last read."""
+    a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    n = ctypes.c_uint32()
+    _check(lib().gms_multisplit_profile(context.handle, ctypes.byref(a), ctypes.byref(b),
+                                        ctypes.byref(c), ctypes.byref(n)), "profile")
+    return a.value, b.value, c.value, n.value
+
+
+def prescan_matrix(d_key_in, num_elements: int, context: multisplit_context, bucket_identifier,
+                   d_matrix, key_value: bool = False) -> None:
+    _check(lib().gms_multisplit_prescan_matrix(context.handle, _ptr(d_key_in), int(num_elements),
+                                               bucket_identifier.desc(), int(bool(key_value)),
+                                               _ptr(d_matrix), _stream(d_key_in)), "prescan_matrix")
+
+
+def exclusive_scan(d_in, d_out, num_elements: int, context: multisplit_context) -> None:
+    _check(lib().gms_exclusive_scan_u32(context.handle, _ptr(d_in), _ptr(d_out), int(num_elements),
+                                        _stream(d_in)), "exclusive_scan")
+
+
+# ---------------------------------------------------------------------------
+# Multisplit radix sort (src/api/multisplit_sort.cuh, multisplit_sort_pairs.cuh)
+# ---------------------------------------------------------------------------
+class ms_sort_context:
+    """ms_sort_context (src/api/multisplit_sort.cuh:100-116)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().gms_sort_create(ctypes.byref(h)), "ms_sort_context")
+        self._h = h
+        self.digit_bits = 0
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.gms_sort_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def multisplit_sort_release_memory(context: ms_sort_context) -> None:
+    """src/api/multisplit_sort.cuh:312-318."""
+    _check(lib().gms_sort_release(context.handle), "multisplit_sort_release_memory")
+
+
+def _sort_api(bits: int):
+    def allocate(n_elements: int, context: ms_sort_context) -> None:
+        _check(lib().gms_sort_allocate(context.handle, int(n_elements), bits, 0),
+               f"multisplit_sort_{bits}bit_allocate")
+        context.digit_bits = bits
+
+    def allocate_pairs(n_elements: int, context: ms_sort_context) -> None:
+        _check(lib().gms_sort_allocate(context.handle, int(n_elements), bits, 1),
+               f"multisplit_sort_{bits}bit_allocate_pairs")
+        context.digit_bits = bits
+
+    def sort(d_key_in, d_key_out, n_elements: int, context: ms_sort_context) -> None:
+        _check_len("d_key_in", d_key_in, n_elements)
+        _check_len("d_key_out", d_key_out, n_elements)
+        _check(lib().gms_sort(context.handle, _ptr(d_key_in), _ptr(d_key_out), None, None,
+                              int(n_elements), bits, _stream(d_key_in, d_key_out)),
+               f"multisplit_sort_{bits}bit")
+
+    def sort_pairs(d_key_in, d_key_out, d_value_in, d_value_out, n_elements: int,
+                   context: ms_sort_context) -> None:
+        for name, t in (("d_key_in", d_key_in), ("d_key_out", d_key_out),
+                        ("d_value_in", d_value_in), ("d_value_out", d_value_out)):
+            _check_len(name, t, n_elements)
+        _check(lib().gms_sort(context.handle, _ptr(d_key_in), _ptr(d_key_out), _ptr(d_value_in),
+                              _ptr(d_value_out), int(n_elements), bits,
+                              _stream(d_key_in, d_key_out)), f"multisplit_sort_{bits}bit_pairs")
+
+    return allocate, allocate_pairs, sort, sort_pairs
+
+
+(multisplit_sort_4bit_allocate, multisplit_sort_4bit_allocate_pairs,
+ multisplit_sort_4bit, multisplit_sort_4bit_pairs) = _sort_api(4)
+(multisplit_sort_5bit_allocate, multisplit_sort_5bit_allocate_pairs,
+ multisplit_sort_5bit, multisplit_sort_5bit_pairs) = _sort_api(5)
+(multisplit_sort_6bit_allocate, multisplit_sort_6bit_allocate_pairs,
+ multisplit_sort_6bit, multisplit_sort_6bit_pairs) = _sort_api(6)
+(multisplit_sort_7bit_allocate, multisplit_sort_7bit_allocate_pairs,
+ multisplit_sort_7bit, multisplit_sort_7bit_pairs) = _sort_api(7)
+(multisplit_sort_8bit_allocate, multisplit_sort_8bit_allocate_pairs,
+ multisplit_sort_8bit, multisplit_sort_8bit_pairs) = _sort_api(8)
+
+
+# ---------------------------------------------------------------------------
+# Compaction (src/kernels/compaction/multisplit2_compaction.cuh:474-573)
+# ---------------------------------------------------------------------------
+class compaction_context:
+    """compaction_context (multisplit2_compaction.cuh:477-489)."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(lib().gms_compaction_create(ctypes.byref(h)), "compaction_context")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.gms_compaction_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def compaction_allocate_key_only(num_elements: int, context: compaction_context) -> None:
+    _check(lib().gms_compaction_allocate(context.handle, int(num_elements), 0),
+           "compaction_allocate_key_only")
+
+
+def compaction_allocate_key_value(num_elements: int, context: compaction_context) -> None:
+    _check(lib().gms_compaction_allocate(context.handle, int(num_elements), 1),
+           "compaction_allocate_key_value")
+
+
+def compaction_release_memory(context: compaction_context) -> None:
+    _check(lib().gms_compaction_release(context.handle), "compaction_release_memory")
+
+
+def compaction_key_only(d_key_in, d_key_out, num_elements: int, context: compaction_context,
+                        bucket_identifier) -> int:
+    """Stable 2-bucket multisplit; returns the bucket-0 count (blocking)."""
+    cnt = ctypes.c_uint32()
+    _check(lib().gms_compaction(context.handle, _ptr(d_key_in), None, _ptr(d_key_out), None,
+                                int(num_elements), bucket_identifier.desc(),
+                                ctypes.addressof(cnt), None, _stream(d_key_in, d_key_out)),
+           "compaction_key_only")
+    return int(cnt.value)
+
+
+def compaction_key_value(d_key_in, d_value_in, d_key_out, d_value_out, num_elements: int,
+                         context: compaction_context, bucket_identifier) -> int:
+    cnt = ctypes.c_uint32()
+    _check(lib().gms_compaction(context.handle, _ptr(d_key_in), _ptr(d_value_in), _ptr(d_key_out),
+                                _ptr(d_value_out), int(num_elements), bucket_identifier.desc(),
+                                ctypes.addressof(cnt), None, _stream(d_key_in, d_key_out)),
+           "compaction_key_value")
+    return int(cnt.value)
+
+
+def compaction_device_count(d_key_in, d_key_out, num_elements: int, context: compaction_context,
+                            bucket_identifier, d_count, d_value_in=None, d_value_out=None) -> None:
+    """Non-blocking compaction: the bucket-0 count goes to device memory."""
+    _check(lib().gms_compaction(context.handle, _ptr(d_key_in), _ptr(d_value_in), _ptr(d_key_out),
+                                _ptr(d_value_out), int(num_elements), bucket_identifier.desc(),
+                                None, _ptr(d_count), _stream(d_key_in, d_key_out)),
+           "compaction")
+
+
+# ---------------------------------------------------------------------------
+# Histogram (src/kernels/histogram/multisplit2_histograms.cuh)
+# ---------------------------------------------------------------------------
+def histogram(d_key_in, num_elements: int, d_histogram, num_bins: int, bucket_identifier) -> None:
+    """Adds bucket-id counts of num_elements keys into d_histogram[num_bins]."""
+    _check(lib().gms_histogram_u32(_ptr(d_key_in), int(num_elements), _ptr(d_histogram),
+                                   int(num_bins), bucket_identifier.desc(), _stream(d_key_in)),
+           "histogram")
+
+
+def multisplit2_histogram_even(d_key_in, num_elements: int, d_histogram, lower_level: float,
+                               upper_level: float, inverse_delta: float, num_bins: int) -> None:
+    """Even-bin float histogram, bin = (uint)((x - lower) * inverse_delta)."""
+    _check(lib().gms_histogram_even_f32(_ptr(d_key_in), int(num_elements), _ptr(d_histogram),
+                                        int(num_bins), float(lower_level), float(upper_level),
+                                        float(inverse_delta), _stream(d_key_in)),
+           "histogram_even")
+
+
+def copy_u32(d_src, d_dst, n: int) -> None:
+    _check(lib().gms_copy_u32(_ptr(d_src), _ptr(d_dst), int(n), _stream(d_src)), "copy_u32")

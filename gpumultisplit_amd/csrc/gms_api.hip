@@ -1,0 +1,613 @@
+// C-ABI entry points (include/gpumultisplit.h) and the host-side engine.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+
+#include "gms_internal.hpp"
+#include "gms_kernels.hpp"
+
+using namespace gms;
+
+struct gms_multisplit_ctx {
+    uint32_t num_buckets = 0;
+    uint32_t capacity = 0;
+    int allocated = 0;
+    Scratch s;
+    Profiler prof;
+};
+
+struct gms_sort_ctx {
+    uint32_t capacity = 0;
+    uint32_t digit_bits = 0;
+    int kv = 0;
+    int allocated = 0;
+    Scratch s;
+    uint32_t* ktmp = nullptr;
+    uint32_t* vtmp = nullptr;
+};
+
+struct gms_compaction_ctx {
+    uint32_t capacity = 0;
+    int allocated = 0;
+    Scratch s;
+};
+
+namespace gms {
+
+gms_status hip_status(hipError_t e) { return e == hipSuccess ? GMS_SUCCESS : GMS_ERR_HIP; }
+
+bool ranges_overlap(const void* a, size_t an, const void* b, size_t bn) {
+    if (!a || !b || an == 0 || bn == 0) return false;
+    const uintptr_t a0 = (uintptr_t)a, a1 = a0 + an, b0 = (uintptr_t)b, b1 = b0 + bn;
+    return a0 < b1 && b0 < a1;
+}
+
+size_t matrix_elems(uint32_t n, uint32_t num_buckets) {
+    if (num_buckets <= 1 || n == 0) return 0;
+    const int lb = ceil_log2_u32(num_buckets);
+    const size_t t_ko = ((size_t)n + tile_keys(lb, false) - 1) / tile_keys(lb, false);
+    const size_t t_kv = ((size_t)n + tile_keys(lb, true) - 1) / tile_keys(lb, true);
+    return ((size_t)1 << lb) * std::max(t_ko, t_kv);
+}
+
+gms_status scratch_alloc(Scratch& s, size_t hist_elems) {
+    scratch_free(s);
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return GMS_ERR_NO_DEVICE;
+    s.device = dev;
+    const size_t he = std::max<size_t>(hist_elems, 4);
+    const size_t pe = scan_partials_needed(he);
+    if (hipMalloc((void**)&s.hist, he * sizeof(uint32_t)) != hipSuccess) {
+        s.hist = nullptr;
+        return GMS_ERR_ALLOC;
+    }
+    if (hipMalloc((void**)&s.part, pe * sizeof(uint32_t)) != hipSuccess) {
+        s.part = nullptr;
+        scratch_free(s);
+        return GMS_ERR_ALLOC;
+    }
+    if (hipMalloc((void**)&s.offs, 256 * sizeof(uint32_t)) != hipSuccess) {
+        s.offs = nullptr;
+        scratch_free(s);
+        return GMS_ERR_ALLOC;
+    }
+    s.hist_cap = he;
+    s.part_cap = pe;
+    return GMS_SUCCESS;
+}
+
+void scratch_free(Scratch& s) {
+    if (s.hist) (void)hipFree(s.hist);
+    if (s.part) (void)hipFree(s.part);
+    if (s.offs) (void)hipFree(s.offs);
+    s.hist = s.part = s.offs = nullptr;
+    s.hist_cap = s.part_cap = 0;
+}
+
+hipEvent_t* Profiler::next(hipStream_t) {
+    if (used + 4 > ev.size()) {
+        if (ev.size() < 4 * 1024) {
+            for (int i = 0; i < 4; ++i) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return nullptr;
+                ev.push_back(e);
+            }
+        } else if (flush() != GMS_SUCCESS) {
+            return nullptr;
+        }
+    }
+    hipEvent_t* r = &ev[used];
+    used += 4;
+    return r;
+}
+
+gms_status Profiler::flush() {
+    for (size_t i = 0; i + 3 < used; i += 4) {
+        if (hipEventSynchronize(ev[i + 3]) != hipSuccess) return GMS_ERR_HIP;
+        for (int st = 0; st < 3; ++st) {
+            float t = 0.f;
+            if (hipEventElapsedTime(&t, ev[i + st], ev[i + st + 1]) != hipSuccess) return GMS_ERR_HIP;
+            ms[st] += t;
+        }
+        ++calls;
+    }
+    used = 0;
+    return GMS_SUCCESS;
+}
+
+void Profiler::destroy() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    ev.clear();
+    used = 0;
+}
+
+static bool is_pow2(uint32_t x) { return x && !(x & (x - 1)); }
+
+gms_status check_desc(uint32_t num_buckets, gms_bucket_desc d) {
+    switch (d.kind) {
+        case GMS_BUCKET_RANGE: return GMS_SUCCESS;
+        case GMS_BUCKET_DELTA: return d.param >= 1 ? GMS_SUCCESS : GMS_ERR_INVALID_ARG;
+        case GMS_BUCKET_DIGIT:
+            if (d.param >= 32 || d.width > 8) return GMS_ERR_INVALID_ARG;
+            if (d.width == 0) return is_pow2(num_buckets) ? GMS_SUCCESS : GMS_ERR_INVALID_ARG;
+            return (1u << d.width) <= num_buckets ? GMS_SUCCESS : GMS_ERR_INVALID_ARG;
+        case GMS_BUCKET_IDENTITY: return GMS_SUCCESS;
+        case GMS_BUCKET_LOWBITS:
+            return (d.param < 32 && (1u << d.param) <= num_buckets) ? GMS_SUCCESS
+                                                                     : GMS_ERR_INVALID_ARG;
+        default: return GMS_ERR_INVALID_ARG;
+    }
+}
+
+static uint32_t digit_mask(uint32_t B, gms_bucket_desc d) {
+    return d.width ? (1u << d.width) - 1u : B - 1u;
+}
+
+template <class Fn>
+static hipError_t with_functor(uint32_t B, gms_bucket_desc d, Fn&& fn) {
+    switch (d.kind) {
+        case GMS_BUCKET_RANGE: return fn(RangeF{B});
+        case GMS_BUCKET_DELTA: return fn(DeltaF::make(d.param, B));
+        case GMS_BUCKET_DIGIT: return fn(DigitF{d.param, digit_mask(B, d)});
+        case GMS_BUCKET_LOWBITS: return fn(DigitF{0u, (1u << d.param) - 1u});
+        case GMS_BUCKET_IDENTITY: return fn(IdentF{B - 1u});
+        default: return hipErrorInvalidValue;
+    }
+}
+
+static bool vec_aligned(const void* p) { return p == nullptr || (((uintptr_t)p) & 15u) == 0; }
+
+gms_status run_prescan_only(Scratch& s, const uint32_t* kin, uint32_t n, uint32_t B,
+                            gms_bucket_desc d, bool kv, hipStream_t stream, uint32_t* nt_out) {
+    const int lb = ceil_log2_u32(B);
+    const uint32_t tile = tile_keys(lb, kv);
+    const uint32_t nt = (uint32_t)(((size_t)n + tile - 1) / tile);
+    if (nt_out) *nt_out = nt;
+    if (n == 0 || B <= 1) return GMS_SUCCESS;
+    if (((size_t)1 << lb) * nt > s.hist_cap) return GMS_ERR_CAPACITY;
+    MsCall c{kin, nullptr, nullptr, nullptr, n, lb, kv, nt, s.hist, vec_aligned(kin), stream};
+    return hip_status(with_functor(B, d, [&](auto f) { return launch_prescan(c, f); }));
+}
+
+gms_status run_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                          uint32_t* vout, uint32_t n, uint32_t B, gms_bucket_desc d,
+                          uint32_t* d_offsets, hipStream_t stream, Profiler* prof) {
+    const bool kv = vin != nullptr;
+    if (n == 0) return d_offsets ? hip_status(launch_fill(d_offsets, 0u, B, stream)) : GMS_SUCCESS;
+    if (B == 1) {
+        hipError_t e = launch_copy(kin, kout, n, stream);
+        if (e == hipSuccess && kv) e = launch_copy(vin, vout, n, stream);
+        if (e == hipSuccess && d_offsets) e = launch_fill(d_offsets, 0u, 1u, stream);
+        return hip_status(e);
+    }
+    const int lb = ceil_log2_u32(B);
+    const uint32_t tile = tile_keys(lb, kv);
+    const uint32_t nt = (uint32_t)(((size_t)n + tile - 1) / tile);
+    const size_t m = ((size_t)1 << lb) * nt;
+    if (m > s.hist_cap || scan_partials_needed(m) > s.part_cap) return GMS_ERR_CAPACITY;
+
+    MsCall c{kin, vin, kout, vout, n, lb, kv, nt, s.hist, vec_aligned(kin) && vec_aligned(vin),
+             stream};
+    hipEvent_t* ev = (prof && prof->on) ? prof->next(stream) : nullptr;
+    hipError_t e = hipSuccess;
+    if (ev) e = hipEventRecord(ev[0], stream);
+    if (e == hipSuccess) e = with_functor(B, d, [&](auto f) { return launch_prescan(c, f); });
+    if (e == hipSuccess && ev) e = hipEventRecord(ev[1], stream);
+    if (e == hipSuccess) e = launch_exclusive_scan(s.hist, s.hist, (uint32_t)m, s.part, stream);
+    if (e == hipSuccess && ev) e = hipEventRecord(ev[2], stream);
+    if (e == hipSuccess) e = with_functor(B, d, [&](auto f) { return launch_postscan(c, f); });
+    if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
+    if (e == hipSuccess && d_offsets) e = launch_gather_offsets(s.hist, nt, B, d_offsets, stream);
+    return hip_status(e);
+}
+
+}  // namespace gms
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* gms_status_string(gms_status s) {
+    switch (s) {
+        case GMS_SUCCESS: return "success";
+        case GMS_ERR_INVALID_ARG: return "invalid argument";
+        case GMS_ERR_UNSUPPORTED_BUCKETS: return "unsupported number of buckets (1..256)";
+        case GMS_ERR_CAPACITY: return "n exceeds the allocated capacity";
+        case GMS_ERR_NOT_ALLOCATED: return "context not allocated";
+        case GMS_ERR_ALLOC: return "device allocation failed";
+        case GMS_ERR_HIP: return "HIP runtime error";
+        case GMS_ERR_ALIAS: return "input and output ranges overlap";
+        case GMS_ERR_NO_DEVICE: return "no HIP device";
+        default: return "unknown status";
+    }
+}
+
+uint32_t gms_abi_version(void) { return GMS_ABI_VERSION; }
+
+gms_status gms_bucket_ids_host(uint32_t num_buckets, gms_bucket_desc desc, const uint32_t* keys,
+                               uint32_t* ids, size_t n) {
+    if (num_buckets < 1 || num_buckets > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    gms_status st = check_desc(num_buckets, desc);
+    if (st != GMS_SUCCESS) return st;
+    if (n && (!keys || !ids)) return GMS_ERR_INVALID_ARG;
+    (void)with_functor(num_buckets, desc, [&](auto f) {
+        for (size_t i = 0; i < n; ++i) ids[i] = num_buckets == 1 ? 0u : f(keys[i]);
+        return hipSuccess;
+    });
+    return GMS_SUCCESS;
+}
+
+uint32_t gms_subproblem_size(uint32_t num_buckets, int key_value) {
+    if (num_buckets < 1 || num_buckets > 256) return 0;
+    return tile_keys(ceil_log2_u32(num_buckets), key_value != 0);
+}
+
+// ---------------------------------------------------------------- multisplit
+gms_status gms_multisplit_create(uint32_t num_buckets, gms_multisplit_ctx** out) {
+    if (!out) return GMS_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (num_buckets < 1 || num_buckets > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    gms_multisplit_ctx* c = new (std::nothrow) gms_multisplit_ctx();
+    if (!c) return GMS_ERR_ALLOC;
+    c->num_buckets = num_buckets;
+    *out = c;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_multisplit_allocate(gms_multisplit_ctx* ctx, uint32_t n_capacity, int key_value) {
+    (void)key_value;  // the matrix is sized for both tile shapes
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    gms_status st = scratch_alloc(ctx->s, matrix_elems(n_capacity, ctx->num_buckets));
+    if (st != GMS_SUCCESS) return st;
+    ctx->capacity = n_capacity;
+    ctx->allocated = 1;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_multisplit_release(gms_multisplit_ctx* ctx) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    scratch_free(ctx->s);
+    ctx->prof.destroy();
+    ctx->allocated = 0;
+    ctx->capacity = 0;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_multisplit_destroy(gms_multisplit_ctx* ctx) {
+    if (!ctx) return GMS_SUCCESS;
+    gms_multisplit_release(ctx);
+    delete ctx;
+    return GMS_SUCCESS;
+}
+
+static gms_status validate_call(gms_multisplit_ctx* ctx, const uint32_t* kin, const uint32_t* vin,
+                                uint32_t* kout, uint32_t* vout, uint32_t n, gms_bucket_desc d) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
+    if (n > ctx->capacity) return GMS_ERR_CAPACITY;
+    gms_status st = check_desc(ctx->num_buckets, d);
+    if (st != GMS_SUCCESS) return st;
+    if (n == 0) return GMS_SUCCESS;
+    if (!kin || !kout) return GMS_ERR_INVALID_ARG;
+    if ((vin == nullptr) != (vout == nullptr)) return GMS_ERR_INVALID_ARG;
+    const size_t bytes = (size_t)n * 4;
+    if (ranges_overlap(kin, bytes, kout, bytes)) return GMS_ERR_ALIAS;
+    if (vin) {
+        if (ranges_overlap(vin, bytes, vout, bytes) || ranges_overlap(kin, bytes, vout, bytes) ||
+            ranges_overlap(vin, bytes, kout, bytes) || ranges_overlap(kout, bytes, vout, bytes))
+            return GMS_ERR_ALIAS;
+    }
+    return GMS_SUCCESS;
+}
+
+gms_status gms_multisplit(gms_multisplit_ctx* ctx, const uint32_t* kin, const uint32_t* vin,
+                          uint32_t* kout, uint32_t* vout, uint32_t n, gms_bucket_desc d,
+                          uint32_t* d_offsets, void* stream) {
+    gms_status st = validate_call(ctx, kin, vin, kout, vout, n, d);
+    if (st != GMS_SUCCESS) return st;
+    return run_multisplit(ctx->s, kin, vin, kout, vout, n, ctx->num_buckets, d, d_offsets,
+                          (hipStream_t)stream, &ctx->prof);
+}
+
+static gms_status multisplit_host_offsets(gms_multisplit_ctx* ctx, const uint32_t* kin,
+                                          const uint32_t* vin, uint32_t* kout, uint32_t* vout,
+                                          uint32_t n, gms_bucket_desc d, uint32_t* h_offsets,
+                                          void* stream) {
+    gms_status st = validate_call(ctx, kin, vin, kout, vout, n, d);
+    if (st != GMS_SUCCESS) return st;
+    hipStream_t s = (hipStream_t)stream;
+    st = run_multisplit(ctx->s, kin, vin, kout, vout, n, ctx->num_buckets, d,
+                        h_offsets ? ctx->s.offs : nullptr, s, &ctx->prof);
+    if (st != GMS_SUCCESS || !h_offsets) return st;
+    if (hipMemcpyAsync(h_offsets, ctx->s.offs, ctx->num_buckets * sizeof(uint32_t),
+                       hipMemcpyDeviceToHost, s) != hipSuccess)
+        return GMS_ERR_HIP;
+    return hip_status(hipStreamSynchronize(s));
+}
+
+gms_status gms_multisplit_key_only(gms_multisplit_ctx* ctx, const uint32_t* kin, uint32_t* kout,
+                                   uint32_t n, gms_bucket_desc d, uint32_t* h_offsets,
+                                   void* stream) {
+    return multisplit_host_offsets(ctx, kin, nullptr, kout, nullptr, n, d, h_offsets, stream);
+}
+
+gms_status gms_multisplit_key_value(gms_multisplit_ctx* ctx, const uint32_t* kin,
+                                    const uint32_t* vin, uint32_t* kout, uint32_t* vout,
+                                    uint32_t n, gms_bucket_desc d, uint32_t* h_offsets,
+                                    void* stream) {
+    if (n && (!vin || !vout)) return GMS_ERR_INVALID_ARG;
+    return multisplit_host_offsets(ctx, kin, vin, kout, vout, n, d, h_offsets, stream);
+}
+
+gms_status gms_multisplit_set_profiling(gms_multisplit_ctx* ctx, int enable) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    ctx->prof.on = enable != 0;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_multisplit_profile(gms_multisplit_ctx* ctx, double* ms_prescan, double* ms_scan,
+                                  double* ms_postscan, uint32_t* calls) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    gms_status st = ctx->prof.flush();
+    if (st != GMS_SUCCESS) return st;
+    if (ms_prescan) *ms_prescan = ctx->prof.ms[0];
+    if (ms_scan) *ms_scan = ctx->prof.ms[1];
+    if (ms_postscan) *ms_postscan = ctx->prof.ms[2];
+    if (calls) *calls = ctx->prof.calls;
+    ctx->prof.ms[0] = ctx->prof.ms[1] = ctx->prof.ms[2] = 0;
+    ctx->prof.calls = 0;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_multisplit_prescan_matrix(gms_multisplit_ctx* ctx, const uint32_t* kin, uint32_t n,
+                                         gms_bucket_desc d, int key_value, uint32_t* d_matrix,
+                                         void* stream) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
+    if (n > ctx->capacity) return GMS_ERR_CAPACITY;
+    gms_status st = check_desc(ctx->num_buckets, d);
+    if (st != GMS_SUCCESS) return st;
+    if (n == 0 || ctx->num_buckets <= 1) return GMS_SUCCESS;
+    if (!kin || !d_matrix) return GMS_ERR_INVALID_ARG;
+    uint32_t nt = 0;
+    hipStream_t s = (hipStream_t)stream;
+    st = run_prescan_only(ctx->s, kin, n, ctx->num_buckets, d, key_value != 0, s, &nt);
+    if (st != GMS_SUCCESS) return st;
+    // Rows past num_buckets (padding to a power of two) are always empty.
+    return hip_status(launch_copy(ctx->s.hist, d_matrix, (size_t)ctx->num_buckets * nt, s));
+}
+
+gms_status gms_exclusive_scan_u32(gms_multisplit_ctx* ctx, const uint32_t* d_in, uint32_t* d_out,
+                                  uint32_t n, void* stream) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
+    if (n == 0) return GMS_SUCCESS;
+    if (!d_in || !d_out) return GMS_ERR_INVALID_ARG;
+    if (scan_partials_needed(n) > ctx->s.part_cap) return GMS_ERR_CAPACITY;
+    if (d_in != d_out && ranges_overlap(d_in, (size_t)n * 4, d_out, (size_t)n * 4))
+        return GMS_ERR_ALIAS;
+    return hip_status(launch_exclusive_scan(d_in, d_out, n, ctx->s.part, (hipStream_t)stream));
+}
+
+// ---------------------------------------------------------------------- sort
+struct SortPass {
+    uint32_t shift, bits;
+};
+
+static int sort_schedule(uint32_t digit_bits, SortPass* p) {
+    int k = 0;
+    switch (digit_bits) {
+        case 8:
+            for (uint32_t s = 0; s < 32; s += 8) p[k++] = {s, 8};
+            break;
+        case 7:
+            p[k++] = {0, 4};
+            for (uint32_t s = 4; s < 32; s += 7) p[k++] = {s, 7};
+            break;
+        case 6:
+            p[k++] = {0, 4};
+            p[k++] = {4, 4};
+            for (uint32_t s = 8; s < 32; s += 6) p[k++] = {s, 6};
+            break;
+        case 5:
+            for (uint32_t s = 0; s < 30; s += 5) p[k++] = {s, 5};
+            p[k++] = {30, 2};
+            break;
+        case 4:
+            for (uint32_t s = 0; s < 32; s += 4) p[k++] = {s, 4};
+            break;
+        default:
+            return 0;
+    }
+    return k;
+}
+
+gms_status gms_sort_create(gms_sort_ctx** out) {
+    if (!out) return GMS_ERR_INVALID_ARG;
+    *out = new (std::nothrow) gms_sort_ctx();
+    return *out ? GMS_SUCCESS : GMS_ERR_ALLOC;
+}
+
+gms_status gms_sort_release(gms_sort_ctx* ctx) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    scratch_free(ctx->s);
+    if (ctx->ktmp) (void)hipFree(ctx->ktmp);
+    if (ctx->vtmp) (void)hipFree(ctx->vtmp);
+    ctx->ktmp = ctx->vtmp = nullptr;
+    ctx->allocated = 0;
+    ctx->capacity = 0;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_sort_allocate(gms_sort_ctx* ctx, uint32_t n_capacity, uint32_t digit_bits,
+                             int key_value) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    SortPass p[8];
+    const int np = sort_schedule(digit_bits, p);
+    if (np == 0) return GMS_ERR_INVALID_ARG;
+    gms_sort_release(ctx);
+    size_t me = 0;
+    for (int i = 0; i < np; ++i) me = std::max(me, matrix_elems(n_capacity, 1u << p[i].bits));
+    gms_status st = scratch_alloc(ctx->s, me);
+    if (st != GMS_SUCCESS) return st;
+    const size_t bytes = std::max<size_t>((size_t)n_capacity, 1) * sizeof(uint32_t);
+    if (hipMalloc((void**)&ctx->ktmp, bytes) != hipSuccess) {
+        ctx->ktmp = nullptr;
+        gms_sort_release(ctx);
+        return GMS_ERR_ALLOC;
+    }
+    if (key_value && hipMalloc((void**)&ctx->vtmp, bytes) != hipSuccess) {
+        ctx->vtmp = nullptr;
+        gms_sort_release(ctx);
+        return GMS_ERR_ALLOC;
+    }
+    ctx->capacity = n_capacity;
+    ctx->digit_bits = digit_bits;
+    ctx->kv = key_value != 0;
+    ctx->allocated = 1;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_sort_destroy(gms_sort_ctx* ctx) {
+    if (!ctx) return GMS_SUCCESS;
+    gms_sort_release(ctx);
+    delete ctx;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_sort(gms_sort_ctx* ctx, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
+                    uint32_t* vout, uint32_t n, uint32_t digit_bits, void* stream) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
+    if (n > ctx->capacity) return GMS_ERR_CAPACITY;
+    SortPass p[8];
+    const int np = sort_schedule(digit_bits, p);
+    if (np == 0) return GMS_ERR_INVALID_ARG;
+    const bool kv = vin != nullptr;
+    if ((vin == nullptr) != (vout == nullptr)) return GMS_ERR_INVALID_ARG;
+    if (kv && !ctx->vtmp) return GMS_ERR_NOT_ALLOCATED;
+    if (n == 0) return GMS_SUCCESS;
+    if (!kin || !kout) return GMS_ERR_INVALID_ARG;
+    const size_t bytes = (size_t)n * 4;
+    if (ranges_overlap(kin, bytes, kout, bytes) ||
+        (kv && (ranges_overlap(vin, bytes, vout, bytes) || ranges_overlap(kin, bytes, vout, bytes) ||
+                ranges_overlap(vin, bytes, kout, bytes) || ranges_overlap(kout, bytes, vout, bytes))))
+        return GMS_ERR_ALIAS;
+    hipStream_t s = (hipStream_t)stream;
+    const uint32_t* sk = kin;
+    const uint32_t* sv = vin;
+    for (int i = 0; i < np; ++i) {
+        // The last pass writes the output; earlier passes alternate so that
+        // the input is never written (multisplit_sort.cuh:320-559).
+        const bool to_out = ((np - 1 - i) % 2) == 0;
+        uint32_t* dk = to_out ? kout : ctx->ktmp;
+        uint32_t* dv = kv ? (to_out ? vout : ctx->vtmp) : nullptr;
+        gms_status st = run_multisplit(ctx->s, sk, sv, dk, dv, n, 1u << p[i].bits,
+                                       gms_bucket_desc{GMS_BUCKET_DIGIT, p[i].shift, p[i].bits}, nullptr, s,
+                                       nullptr);
+        if (st != GMS_SUCCESS) return st;
+        sk = dk;
+        sv = dv;
+    }
+    return GMS_SUCCESS;
+}
+
+// ---------------------------------------------------------------- compaction
+gms_status gms_compaction_create(gms_compaction_ctx** out) {
+    if (!out) return GMS_ERR_INVALID_ARG;
+    *out = new (std::nothrow) gms_compaction_ctx();
+    return *out ? GMS_SUCCESS : GMS_ERR_ALLOC;
+}
+
+gms_status gms_compaction_allocate(gms_compaction_ctx* ctx, uint32_t n_capacity, int key_value) {
+    (void)key_value;
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    gms_status st = scratch_alloc(ctx->s, matrix_elems(n_capacity, 2));
+    if (st != GMS_SUCCESS) return st;
+    ctx->capacity = n_capacity;
+    ctx->allocated = 1;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_compaction_release(gms_compaction_ctx* ctx) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    scratch_free(ctx->s);
+    ctx->allocated = 0;
+    ctx->capacity = 0;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_compaction_destroy(gms_compaction_ctx* ctx) {
+    if (!ctx) return GMS_SUCCESS;
+    gms_compaction_release(ctx);
+    delete ctx;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_compaction(gms_compaction_ctx* ctx, const uint32_t* kin, const uint32_t* vin,
+                          uint32_t* kout, uint32_t* vout, uint32_t n, gms_bucket_desc d,
+                          uint32_t* h_count, uint32_t* d_count, void* stream) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
+    if (n > ctx->capacity) return GMS_ERR_CAPACITY;
+    gms_status st = check_desc(2, d);
+    if (st != GMS_SUCCESS) return st;
+    if ((vin == nullptr) != (vout == nullptr)) return GMS_ERR_INVALID_ARG;
+    if (n && (!kin || !kout)) return GMS_ERR_INVALID_ARG;
+    const size_t bytes = (size_t)n * 4;
+    if (ranges_overlap(kin, bytes, kout, bytes) ||
+        (vin && (ranges_overlap(vin, bytes, vout, bytes) || ranges_overlap(kin, bytes, vout, bytes) ||
+                 ranges_overlap(vin, bytes, kout, bytes) || ranges_overlap(kout, bytes, vout, bytes))))
+        return GMS_ERR_ALIAS;
+    hipStream_t s = (hipStream_t)stream;
+    // Offsets {0, count0} land in the scratch's offset slots; with n == 0 they are zero.
+    st = run_multisplit(ctx->s, kin, vin, kout, vout, n, 2, d, ctx->s.offs, s, nullptr);
+    if (st != GMS_SUCCESS) return st;
+    if (d_count &&
+        hipMemcpyAsync(d_count, ctx->s.offs + 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s) !=
+            hipSuccess)
+        return GMS_ERR_HIP;
+    if (h_count) {
+        if (hipMemcpyAsync(h_count, ctx->s.offs + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s) !=
+            hipSuccess)
+            return GMS_ERR_HIP;
+        return hip_status(hipStreamSynchronize(s));
+    }
+    return GMS_SUCCESS;
+}
+
+// ----------------------------------------------------------------- histogram
+gms_status gms_histogram_u32(const uint32_t* kin, uint32_t n, uint32_t* d_hist, uint32_t num_bins,
+                             gms_bucket_desc d, void* stream) {
+    if (num_bins < 1 || num_bins > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    gms_status st = check_desc(num_bins, d);
+    if (st != GMS_SUCCESS) return st;
+    if (!d_hist) return GMS_ERR_INVALID_ARG;
+    if (n == 0) return GMS_SUCCESS;
+    if (!kin) return GMS_ERR_INVALID_ARG;
+    return hip_status(launch_histogram_u32(kin, n, d_hist, num_bins, d, (hipStream_t)stream));
+}
+
+gms_status gms_histogram_even_f32(const float* x, uint32_t n, int32_t* d_hist, uint32_t num_bins,
+                                  float lower, float upper, float inv_delta, void* stream) {
+    (void)upper;  // the reference passes it but bins only with lower and inv_delta
+    if (num_bins < 1 || num_bins > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    if (!d_hist) return GMS_ERR_INVALID_ARG;
+    if (n == 0) return GMS_SUCCESS;
+    if (!x) return GMS_ERR_INVALID_ARG;
+    return hip_status(
+        launch_histogram_even_f32(x, n, d_hist, num_bins, lower, inv_delta, (hipStream_t)stream));
+}
+
+// ------------------------------------------------------------------- utility
+gms_status gms_copy_u32(const uint32_t* src, uint32_t* dst, size_t n, void* stream) {
+    if (n == 0) return GMS_SUCCESS;
+    if (!src || !dst) return GMS_ERR_INVALID_ARG;
+    if (ranges_overlap(src, n * 4, dst, n * 4)) return GMS_ERR_ALIAS;
+    return hip_status(launch_copy(src, dst, n, (hipStream_t)stream));
+}
+
+}  // extern "C"

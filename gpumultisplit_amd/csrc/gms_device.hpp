@@ -1,0 +1,123 @@
+// Device building blocks for the gfx950 multisplit kernels: bucket
+// identifiers, wave64 scans on DPP, and the XCD-aware tile order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GMS_HD __host__ __device__ __forceinline__
+
+namespace gms {
+
+// ---------------------------------------------------------------------------
+// Bucket identifiers.  Host+device so that the host check entry point runs
+// exactly the integer code the kernels run.  Every identifier returns an id in
+// [0, B): the ones that cannot overflow by construction (range, digit) do no
+// clamping, the others clamp into B-1.
+// ---------------------------------------------------------------------------
+
+// floor(key * B / 2^32): the range identifier of the plain-multisplit sweep.
+struct RangeF {
+    uint32_t B;
+    GMS_HD uint32_t operator()(uint32_t k) const {
+        return (uint32_t)(((uint64_t)k * (uint64_t)B) >> 32);
+    }
+};
+
+// key / delta (delta_func, src/main/main_multisplit.cu:41-47) by multiplication
+// with a precomputed reciprocal: for 1 <= d < 2^32 let l = ceil(log2 d) and
+// m = floor(2^32 * (2^l - d) / d) + 1; then for every 32-bit k
+//   q = (t + ((k - t) >> min(l,1))) >> max(l-1,0),  t = mulhi(m, k)
+// is exactly floor(k / d) (Granlund & Montgomery, "Division by invariant
+// integers using multiplication", PLDI 1994, section 4).
+struct DeltaF {
+    uint32_t m, sh1, sh2, bmax;
+    static DeltaF make(uint32_t d, uint32_t num_buckets) {
+        uint32_t l = 0;
+        while (l < 32 && (1ull << l) < (uint64_t)d) ++l;
+        DeltaF f;
+        f.m = (uint32_t)(((((uint64_t)1 << l) - d) << 32) / d + 1);
+        f.sh1 = l < 1 ? l : 1;
+        f.sh2 = l > 0 ? l - 1 : 0;
+        f.bmax = num_buckets - 1;
+        return f;
+    }
+    GMS_HD uint32_t operator()(uint32_t k) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        uint32_t t = __umulhi(m, k);
+#else
+        uint32_t t = (uint32_t)(((uint64_t)m * k) >> 32);
+#endif
+        uint32_t q = (t + ((k - t) >> sh1)) >> sh2;
+        return q < bmax ? q : bmax;
+    }
+};
+
+// (key >> shift) & mask with mask = B-1, B a power of two
+// (ms_bucket_sort_Nbit<x>, src/api/multisplit_sort.cuh:63-98, and
+// test_bucket<flag>, src/main/main_compaction.cu:27-32, with shift 0).
+struct DigitF {
+    uint32_t shift, mask;
+    GMS_HD uint32_t operator()(uint32_t k) const { return (k >> shift) & mask; }
+};
+
+// key itself (identity_bucket, src/main/main_multisplit.cu:34-39).
+struct IdentF {
+    uint32_t bmax;
+    GMS_HD uint32_t operator()(uint32_t k) const { return k < bmax ? k : bmax; }
+};
+
+// ---------------------------------------------------------------------------
+// Wave64 scans on DPP.  Rows are 16 lanes: four row_shr steps give prefixes
+// inside a row, row_bcast:15 / row_bcast:31 carry them across rows.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return x;
+}
+
+// Value of lane 63 broadcast to the wave (the total after an inclusive scan).
+__device__ __forceinline__ uint32_t wave_last(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
+// Exclusive scan across a block of NW waves; `total` receives the block sum.
+// s_w must hold NW words; the call contains one __syncthreads and callers
+// must separate consecutive uses of the same s_w by a barrier.
+template <int NW>
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* s_w, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t inc = wave_incl_scan(x);
+    if (lane == 63u) s_w[wid] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < NW; ++i) {
+        uint32_t v = s_w[i];
+        pre += ((uint32_t)i < wid) ? v : 0u;
+        tot += v;
+    }
+    total = tot;
+    return pre + inc - x;
+}
+
+// ---------------------------------------------------------------------------
+// Tile order.  Workgroups are handed to the 8 XCDs round-robin (workgroup i
+// runs on XCD i mod 8), each XCD with its own L2.  Remap so that every XCD
+// walks one contiguous range of tiles: neighbouring tiles then share L2 lines
+// of the histogram matrix and of the per-bucket output runs.  A bijection on
+// [0, num_tiles) for any num_tiles.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t bid, uint32_t num_tiles) {
+    constexpr uint32_t NX = 8;
+    const uint32_t q = num_tiles / NX, r = num_tiles % NX;
+    const uint32_t x = bid % NX, i = bid / NX;
+    return x * q + (x < r ? x : r) + i;
+}
+
+}  // namespace gms

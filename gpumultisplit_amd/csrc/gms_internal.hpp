@@ -1,0 +1,95 @@
+// Host-side engine shared by the C-ABI entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/gpumultisplit.h"
+#include "gms_device.hpp"
+#include "gms_tune.hpp"
+
+namespace gms {
+
+// Arguments of one plain multisplit launch sequence.
+struct MsCall {
+    const uint32_t* kin;
+    const uint32_t* vin;  // NULL for key-only
+    uint32_t* kout;
+    uint32_t* vout;
+    uint32_t n;
+    int log_b;
+    bool kv;
+    uint32_t num_tiles;
+    uint32_t* H;  // histogram matrix, num_buckets(pow2 of log_b) * num_tiles
+    int vec_ok;
+    hipStream_t stream;
+};
+
+// Functor-family launchers (one translation unit per family).
+hipError_t launch_prescan(const MsCall& c, const RangeF& f);
+hipError_t launch_prescan(const MsCall& c, const DeltaF& f);
+hipError_t launch_prescan(const MsCall& c, const DigitF& f);
+hipError_t launch_prescan(const MsCall& c, const IdentF& f);
+hipError_t launch_postscan(const MsCall& c, const RangeF& f);
+hipError_t launch_postscan(const MsCall& c, const DeltaF& f);
+hipError_t launch_postscan(const MsCall& c, const DigitF& f);
+hipError_t launch_postscan(const MsCall& c, const IdentF& f);
+
+// Exclusive scan of m u32 (in may equal out); part holds scan_partials_needed(m).
+hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t m, uint32_t* part,
+                                 hipStream_t stream);
+size_t scan_partials_needed(size_t m);
+hipError_t launch_gather_offsets(const uint32_t* G, uint32_t num_tiles, uint32_t num_buckets,
+                                 uint32_t* out, hipStream_t stream);
+hipError_t launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t stream);
+hipError_t launch_copy(const uint32_t* src, uint32_t* dst, size_t n, hipStream_t stream);
+hipError_t launch_histogram_u32(const uint32_t* kin, uint32_t n, uint32_t* d_hist, uint32_t nbins,
+                                gms_bucket_desc d, hipStream_t stream);
+hipError_t launch_histogram_even_f32(const float* x, uint32_t n, int32_t* d_hist, uint32_t nbins,
+                                     float lower, float inv_delta, hipStream_t stream);
+
+// Device scratch of a multisplit: the histogram matrix and the scan partials.
+struct Scratch {
+    uint32_t* hist = nullptr;
+    size_t hist_cap = 0;  // u32 elements
+    uint32_t* part = nullptr;
+    size_t part_cap = 0;
+    uint32_t* offs = nullptr;  // 256 bucket starts (staging for host offsets)
+    int device = -1;
+};
+
+gms_status scratch_alloc(Scratch& s, size_t hist_elems);
+void scratch_free(Scratch& s);
+// Matrix elements needed for n keys in B buckets (key-only and key-value tiles).
+size_t matrix_elems(uint32_t n, uint32_t num_buckets);
+
+// Per-call stage events (prescan, scan, postscan) when profiling is on.
+struct Profiler {
+    bool on = false;
+    std::vector<hipEvent_t> ev;  // 4 per recorded call
+    size_t used = 0;             // events in use
+    double ms[3] = {0, 0, 0};
+    uint32_t calls = 0;
+    hipEvent_t* next(hipStream_t s);  // 4 fresh events (flushes when full)
+    gms_status flush();
+    void destroy();
+};
+
+gms_status check_desc(uint32_t num_buckets, gms_bucket_desc d);
+
+// Full prescan -> scan -> postscan on the given scratch.  d_offsets (device,
+// num_buckets entries) optional.  Validation is the caller's job.
+gms_status run_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                          uint32_t* vout, uint32_t n, uint32_t num_buckets, gms_bucket_desc d,
+                          uint32_t* d_offsets, hipStream_t stream, Profiler* prof);
+
+// Prescan only (for the stage-level check).
+gms_status run_prescan_only(Scratch& s, const uint32_t* kin, uint32_t n, uint32_t num_buckets,
+                            gms_bucket_desc d, bool kv, hipStream_t stream, uint32_t* num_tiles);
+
+gms_status hip_status(hipError_t e);
+bool ranges_overlap(const void* a, size_t an, const void* b, size_t bn);
+
+}  // namespace gms

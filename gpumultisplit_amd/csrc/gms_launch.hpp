@@ -1,0 +1,56 @@
+// Launch helpers instantiated once per bucket-identifier family (inst_*.hip).
+#pragma once
+
+#include "gms_internal.hpp"
+#include "gms_kernels.hpp"
+
+namespace gms {
+
+template <int LOG_B, bool KV, class F>
+hipError_t launch_prescan_t(const MsCall& c, const F& f) {
+    constexpr int K = tune_items(LOG_B, KV);
+    hipLaunchKernelGGL((ms_prescan_kernel<LOG_B, K, F>), dim3(c.num_tiles), dim3(kThreads), 0,
+                       c.stream, c.kin, c.n, c.num_tiles, c.H, f, c.vec_ok);
+    return hipGetLastError();
+}
+
+template <int LOG_B, bool KV, class F>
+hipError_t launch_postscan_t(const MsCall& c, const F& f) {
+    constexpr int K = tune_items(LOG_B, KV);
+    constexpr int LB = tune_level_bits(LOG_B);
+    hipLaunchKernelGGL((ms_postscan_kernel<LOG_B, K, LB, KV, F>), dim3(c.num_tiles),
+                       dim3(kThreads), 0, c.stream, c.kin, c.vin, c.kout, c.vout, c.n,
+                       c.num_tiles, c.H, f, c.vec_ok);
+    return hipGetLastError();
+}
+
+#define GMS_SWITCH_LOGB_KV(FN, c, f)                                   \
+    switch ((c).log_b * 2 + ((c).kv ? 1 : 0)) {                        \
+        case 2: return FN<1, false>(c, f);                             \
+        case 3: return FN<1, true>(c, f);                              \
+        case 4: return FN<2, false>(c, f);                             \
+        case 5: return FN<2, true>(c, f);                              \
+        case 6: return FN<3, false>(c, f);                             \
+        case 7: return FN<3, true>(c, f);                              \
+        case 8: return FN<4, false>(c, f);                             \
+        case 9: return FN<4, true>(c, f);                              \
+        case 10: return FN<5, false>(c, f);                            \
+        case 11: return FN<5, true>(c, f);                             \
+        case 12: return FN<6, false>(c, f);                            \
+        case 13: return FN<6, true>(c, f);                             \
+        case 14: return FN<7, false>(c, f);                            \
+        case 15: return FN<7, true>(c, f);                             \
+        case 16: return FN<8, false>(c, f);                            \
+        case 17: return FN<8, true>(c, f);                             \
+        default: return hipErrorInvalidValue;                          \
+    }
+
+#define GMS_DEFINE_LAUNCHERS(FT)                                                         \
+    hipError_t launch_prescan(const MsCall& c, const FT& f) {                            \
+        GMS_SWITCH_LOGB_KV(launch_prescan_t, c, f)                                       \
+    }                                                                                    \
+    hipError_t launch_postscan(const MsCall& c, const FT& f) {                           \
+        GMS_SWITCH_LOGB_KV(launch_postscan_t, c, f)                                      \
+    }
+
+}  // namespace gms

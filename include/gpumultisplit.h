@@ -1,0 +1,234 @@
+/*
+ * gpumultisplit.h -- C ABI of the MI355X (gfx950) multisplit library.
+ *
+ * libgpumultisplit.so exports exactly the functions declared here.  They carry
+ * plain pointers and sizes only: device pointers are hipMalloc'd memory owned
+ * by the caller, `stream` is a hipStream_t passed as void* (NULL = the default
+ * stream), host pointers are ordinary host memory.  Every function returns a
+ * gms_status; none throws.  The C++ header gpumultisplit.hpp rebuilds the
+ * reference's C++ API (multisplit_context, multisplit_key_value, ...) on top.
+ *
+ * Each declaration names the reference interface it replaces (file:line in
+ * the reference repository).
+ */
+#ifndef GPUMULTISPLIT_H_
+#define GPUMULTISPLIT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GMS_ABI_VERSION 1
+
+typedef enum gms_status {
+    GMS_SUCCESS = 0,
+    GMS_ERR_INVALID_ARG = 1,        /* null/aliased pointer, bad descriptor     */
+    GMS_ERR_UNSUPPORTED_BUCKETS = 2,/* bucket count outside [1, 256]            */
+    GMS_ERR_CAPACITY = 3,           /* n larger than the allocated capacity     */
+    GMS_ERR_NOT_ALLOCATED = 4,      /* call before *_allocate                   */
+    GMS_ERR_ALLOC = 5,              /* device allocation failed                 */
+    GMS_ERR_HIP = 6,                /* a HIP runtime call failed                */
+    GMS_ERR_ALIAS = 7,              /* input and output ranges overlap          */
+    GMS_ERR_NO_DEVICE = 8           /* no HIP device visible                    */
+} gms_status;
+
+/* Bucket identifiers.  A functor object cannot cross a C ABI, so the built-in
+ * identifiers of the reference are enumerated; `param` is their one runtime
+ * parameter.  Ids that fall outside [0, B) -- undefined behaviour in the
+ * reference -- are clamped into bucket B-1. */
+typedef enum gms_bucket_kind {
+    GMS_BUCKET_RANGE = 0,     /* floor(key * B / 2^32); param unused            */
+    GMS_BUCKET_DELTA = 1,     /* key / param           (delta_func,
+                                 src/main/main_multisplit.cu:41-47)           */
+    GMS_BUCKET_DIGIT = 2,     /* (key >> param) & (2^width - 1); width 0 means
+                                 log2(B) for a power-of-two B; 2^width <= B
+                                 (ms_bucket_sort_Nbit<x>,
+                                 src/api/multisplit_sort.cuh:63-98)           */
+    GMS_BUCKET_IDENTITY = 3,  /* key                   (identity_bucket,
+                                 src/main/main_multisplit.cu:34-39)           */
+    GMS_BUCKET_LOWBITS = 4    /* key & ((1<<param)-1)  (test_bucket<flag>,
+                                 src/main/main_compaction.cu:27-32)           */
+} gms_bucket_kind;
+
+typedef struct gms_bucket_desc {
+    uint32_t kind;    /* gms_bucket_kind */
+    uint32_t param;   /* delta, shift or flag */
+    uint32_t width;   /* digit width in bits (GMS_BUCKET_DIGIT only) */
+} gms_bucket_desc;
+
+/* Short description of a status code (static storage). */
+const char *gms_status_string(gms_status s);
+
+/* ABI version of the loaded library (GMS_ABI_VERSION it was built with). */
+uint32_t gms_abi_version(void);
+
+/* Bucket ids evaluated on the host with the same integer code the kernels run
+ * (used to check the magic-number division of GMS_BUCKET_DELTA on the CPU). */
+gms_status gms_bucket_ids_host(uint32_t num_buckets, gms_bucket_desc desc,
+                               const uint32_t *keys, uint32_t *ids, size_t n);
+
+/* ------------------------------------------------------------------------ */
+/* Multisplit                                                                */
+/* ------------------------------------------------------------------------ */
+
+typedef struct gms_multisplit_ctx gms_multisplit_ctx;
+
+/* multisplit_context(uint32_t n_buckets) (src/api/multisplit.cuh:29-47).
+ * 1 <= num_buckets <= 256 (the reference accepts powers of two 2..256). */
+gms_status gms_multisplit_create(uint32_t num_buckets, gms_multisplit_ctx **out);
+
+/* multisplit_allocate_key_only(n, ctx)  (src/api/multisplit.cuh:50-61) with
+ * key_value = 0, multisplit_allocate_key_value(n, ctx) (:63-74) with 1.
+ * Allocates the per-tile histogram matrix and scan scratch for any later call
+ * with n <= n_capacity (the reference freezes the grid at this n). */
+gms_status gms_multisplit_allocate(gms_multisplit_ctx *ctx, uint32_t n_capacity,
+                                   int key_value);
+
+/* multisplit_release_memory(ctx) (src/api/multisplit.cuh:76-79). */
+gms_status gms_multisplit_release(gms_multisplit_ctx *ctx);
+
+/* Frees the context object itself (the reference's destructor is a no-op). */
+gms_status gms_multisplit_destroy(gms_multisplit_ctx *ctx);
+
+/* multisplit_key_only(d_key_in, d_key_out, n, ctx, f, h_offsets)
+ * (src/api/multisplit.cuh:81-197).  Stable.  If h_offsets != NULL it receives
+ * the B global bucket starts and the call blocks until the result is ready;
+ * otherwise the call is asynchronous on `stream`. */
+gms_status gms_multisplit_key_only(gms_multisplit_ctx *ctx, const uint32_t *d_key_in,
+                                   uint32_t *d_key_out, uint32_t n,
+                                   gms_bucket_desc desc, uint32_t *h_offsets,
+                                   void *stream);
+
+/* multisplit_key_value(d_key_in, d_value_in, d_key_out, d_value_out, n, ctx, f,
+ * h_offsets) (src/api/multisplit.cuh:199-313).  Values are 32-bit payloads. */
+gms_status gms_multisplit_key_value(gms_multisplit_ctx *ctx, const uint32_t *d_key_in,
+                                    const uint32_t *d_value_in, uint32_t *d_key_out,
+                                    uint32_t *d_value_out, uint32_t n,
+                                    gms_bucket_desc desc, uint32_t *h_offsets,
+                                    void *stream);
+
+/* Same as the two above (d_value_in/out NULL = key-only) but writes the B
+ * bucket starts to device memory d_offsets without blocking.  Replaces the
+ * reference's B-1 blocking 4-byte copies (src/api/multisplit.cuh:144-149). */
+gms_status gms_multisplit(gms_multisplit_ctx *ctx, const uint32_t *d_key_in,
+                          const uint32_t *d_value_in, uint32_t *d_key_out,
+                          uint32_t *d_value_out, uint32_t n, gms_bucket_desc desc,
+                          uint32_t *d_offsets, void *stream);
+
+/* subproblem_size_bms_key_only / _key_value (src/api/bms_api.h:23-89): keys
+ * per tile for a bucket count.  Returns 0 for an unsupported count. */
+uint32_t gms_subproblem_size(uint32_t num_buckets, int key_value);
+
+/* Stage timing for measurement: when enabled, every multisplit on this context
+ * records HIP events on its stream around the prescan, scan and postscan
+ * kernels.  gms_multisplit_profile synchronises and returns the summed
+ * milliseconds and the number of recorded calls since the last reset. */
+gms_status gms_multisplit_set_profiling(gms_multisplit_ctx *ctx, int enable);
+gms_status gms_multisplit_profile(gms_multisplit_ctx *ctx, double *ms_prescan,
+                                  double *ms_scan, double *ms_postscan,
+                                  uint32_t *calls);
+
+/* Stage-level check: runs only the prescan on d_key_in and copies the per-tile
+ * bucket counts, bucket-major (H[b * tiles + t]), to d_matrix (B * tiles u32,
+ * tiles = ceil(n / gms_subproblem_size)).  Same quantity the reference's
+ * cpu_prescan_test checks (src/cpu_functions.h:55-103). */
+gms_status gms_multisplit_prescan_matrix(gms_multisplit_ctx *ctx, const uint32_t *d_key_in,
+                                         uint32_t n, gms_bucket_desc desc, int key_value,
+                                         uint32_t *d_matrix, void *stream);
+
+/* Device-wide exclusive prefix sum of n u32 (d_in == d_out allowed), the scan
+ * stage that replaces cub::DeviceScan::ExclusiveSum (src/api/multisplit.cuh:141,
+ * :258).  Uses the context's scratch; n <= the context's matrix capacity. */
+gms_status gms_exclusive_scan_u32(gms_multisplit_ctx *ctx, const uint32_t *d_in,
+                                  uint32_t *d_out, uint32_t n, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Multisplit radix sort                                                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct gms_sort_ctx gms_sort_ctx;
+
+/* ms_sort_context (src/api/multisplit_sort.cuh:100-116). */
+gms_status gms_sort_create(gms_sort_ctx **out);
+
+/* multisplit_sort_{4,5,6,7,8}bit_allocate[_pairs](n, ctx)
+ * (src/api/multisplit_sort.cuh:118-309).  digit_bits in {4,5,6,7,8}. */
+gms_status gms_sort_allocate(gms_sort_ctx *ctx, uint32_t n_capacity, uint32_t digit_bits,
+                             int key_value);
+
+/* multisplit_sort_release_memory (src/api/multisplit_sort.cuh:312-318). */
+gms_status gms_sort_release(gms_sort_ctx *ctx);
+gms_status gms_sort_destroy(gms_sort_ctx *ctx);
+
+/* multisplit_sort_{4..8}bit(d_key_in, d_key_out, n, ctx)
+ * (src/api/multisplit_sort.cuh:320-559) and
+ * multisplit_sort_{4..8}bit_pairs(d_key_in, d_key_out, d_value_in, d_value_out, n, ctx)
+ * (src/api/multisplit_sort_pairs.cuh:21-260) -- note the reference's argument
+ * order: keys in, keys out, values in, values out.  Values NULL = key-only.
+ * LSD radix sort over the reference's digit schedule; d_key_in is not written. */
+gms_status gms_sort(gms_sort_ctx *ctx, const uint32_t *d_key_in, uint32_t *d_key_out,
+                    const uint32_t *d_value_in, uint32_t *d_value_out, uint32_t n,
+                    uint32_t digit_bits, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Compaction (stable 2-bucket multisplit returning the bucket-0 count)      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct gms_compaction_ctx gms_compaction_ctx;
+
+/* compaction_context (src/kernels/compaction/multisplit2_compaction.cuh:477-489). */
+gms_status gms_compaction_create(gms_compaction_ctx **out);
+
+/* compaction_allocate_key_only / _key_value (multisplit2_compaction.cuh:491-517). */
+gms_status gms_compaction_allocate(gms_compaction_ctx *ctx, uint32_t n_capacity, int key_value);
+
+/* compaction_release_memory (multisplit2_compaction.cuh:519-523). */
+gms_status gms_compaction_release(gms_compaction_ctx *ctx);
+gms_status gms_compaction_destroy(gms_compaction_ctx *ctx);
+
+/* compaction_key_only(kin, kout, n, ctx, f) -> unsigned bucket-0 count and
+ * compaction_key_value(kin, vin, kout, vout, n, ctx, f)
+ * (multisplit2_compaction.cuh:525-573).  The 2-bucket descriptor must give
+ * ids in {0,1} (others clamp to 1).  If h_count != NULL the call blocks and
+ * stores the count there (the reference's behaviour); if d_count != NULL the
+ * count is written to device memory asynchronously.  Values NULL = key-only. */
+gms_status gms_compaction(gms_compaction_ctx *ctx, const uint32_t *d_key_in,
+                          const uint32_t *d_value_in, uint32_t *d_key_out,
+                          uint32_t *d_value_out, uint32_t n, gms_bucket_desc desc,
+                          uint32_t *h_count, uint32_t *d_count, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Histogram                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Adds the counts of the bucket ids of n keys into d_hist[num_bins] (the
+ * caller zeroes it, as with the reference's histogram kernels).  Replaces
+ * multisplit2_histogram_even_{128,256} and _{64,128,256}bin_128 over integer
+ * bins (src/kernels/histogram/multisplit2_histograms.cuh:30-392). */
+gms_status gms_histogram_u32(const uint32_t *d_key_in, uint32_t n, uint32_t *d_hist,
+                             uint32_t num_bins, gms_bucket_desc desc, void *stream);
+
+/* Even-bin histogram of float samples, bin = (uint32)((x - lower) * inv_delta)
+ * in float32 (multisplit2_histogram_even_*(KeyT*, int n, int* d_hist, lower,
+ * upper, inv_delta), multisplit2_histograms.cuh:36-392, bin formula :60).
+ * Adds into d_hist[num_bins]; samples outside [0, num_bins) are ignored. */
+gms_status gms_histogram_even_f32(const float *d_samples, uint32_t n, int32_t *d_hist,
+                                  uint32_t num_bins, float lower, float upper,
+                                  float inv_delta, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Utilities                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Plain device copy of n u32 with the library's streaming kernel (measures
+ * the copy bandwidth that HBM counters are calibrated against). */
+gms_status gms_copy_u32(const uint32_t *d_src, uint32_t *d_dst, size_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GPUMULTISPLIT_H_ */

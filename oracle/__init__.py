@@ -1,0 +1,146 @@
+"""CPU oracle for the multisplit hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker (or, for
+``cpu_baseline``, as the timed CPU restatement).  The product package
+``gpumultisplit_amd`` never imports it.
+
+numpy wrappers over ``oracle/liboracle.so`` (built from ``oracle/oracle.c`` by
+``oracle/Makefile``).  The C file is a clean-room restatement of the
+reference's CPU multisplit (src/cpu_functions.h:155-288), its sort digit
+schedules (src/api/multisplit_sort.cuh:320-559), histogram semantics
+(src/kernels/histogram/multisplit2_histograms.cuh:36-392) and compaction
+return value (src/kernels/compaction/multisplit2_compaction.cuh:525-573).
+
+Pinning: every function is checked in tests/test_oracle.py against an
+independent computation (numpy stable sorts, bincount, Python integer
+arithmetic, brute-force closed forms on tiny inputs).  No function is left
+"parity unpinned".
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+RANGE, DELTA, DIGIT, IDENTITY, LOWBITS = 0, 1, 2, 3, 4
+
+_lib = None
+
+
+def build() -> None:
+    """Compile oracle/oracle.c into oracle/liboracle.so."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        u32p = ctypes.c_void_p
+        L.ms_oracle_bucket_ids.argtypes = [ctypes.c_uint32] * 3 + [u32p, ctypes.c_uint64, u32p]
+        L.ms_oracle_bucket_ids.restype = None
+        L.ms_oracle_multisplit.argtypes = [ctypes.c_uint32] * 3 + [u32p] * 4 + [ctypes.c_uint64, u32p]
+        L.ms_oracle_multisplit.restype = ctypes.c_int
+        L.ms_oracle_sort_schedule.argtypes = [ctypes.c_uint32, u32p, u32p]
+        L.ms_oracle_sort_schedule.restype = ctypes.c_int
+        L.ms_oracle_sort.argtypes = [u32p] * 4 + [ctypes.c_uint64, ctypes.c_uint32]
+        L.ms_oracle_sort.restype = ctypes.c_int
+        L.ms_oracle_histogram.argtypes = [ctypes.c_uint32] * 3 + [u32p, ctypes.c_uint64, u32p]
+        L.ms_oracle_histogram.restype = None
+        L.ms_oracle_histogram_even_f32.argtypes = [u32p, ctypes.c_uint64, u32p, ctypes.c_uint32,
+                                                   ctypes.c_float, ctypes.c_float]
+        L.ms_oracle_histogram_even_f32.restype = None
+        L.ms_oracle_compaction.argtypes = [ctypes.c_uint32] * 2 + [u32p] * 4 + [ctypes.c_uint64]
+        L.ms_oracle_compaction.restype = ctypes.c_uint64
+        L.ms_oracle_tile_histograms.argtypes = [ctypes.c_uint32] * 3 + [u32p, ctypes.c_uint64,
+                                                                        ctypes.c_uint32, u32p]
+        L.ms_oracle_tile_histograms.restype = None
+        _lib = L
+    return _lib
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+def bucket_ids(keys, num_buckets: int, kind: int = RANGE, param: int = 0) -> np.ndarray:
+    keys = _u32(keys)
+    ids = np.empty_like(keys)
+    lib().ms_oracle_bucket_ids(kind, num_buckets, param, _ptr(keys), keys.size, _ptr(ids))
+    return ids
+
+
+def multisplit(keys, num_buckets: int, kind: int = RANGE, param: int = 0, values=None):
+    """Stable multisplit.  Returns (keys_out, values_out or None, offsets[B])."""
+    keys = _u32(keys)
+    vals = _u32(values) if values is not None else None
+    kout = np.empty_like(keys)
+    vout = np.empty_like(vals) if vals is not None else None
+    offs = np.zeros(num_buckets, dtype=np.uint32)
+    rc = lib().ms_oracle_multisplit(kind, num_buckets, param, _ptr(keys), _ptr(vals),
+                                    _ptr(kout), _ptr(vout), keys.size, _ptr(offs))
+    if rc != 0:
+        raise MemoryError("oracle multisplit failed")
+    return kout, vout, offs
+
+
+def sort_schedule(digit_bits: int):
+    s = np.zeros(8, dtype=np.uint32)
+    b = np.zeros(8, dtype=np.uint32)
+    p = lib().ms_oracle_sort_schedule(digit_bits, _ptr(s), _ptr(b))
+    return [(int(s[i]), int(b[i])) for i in range(p)]
+
+
+def sort(keys, digit_bits: int = 8, values=None):
+    keys = _u32(keys)
+    vals = _u32(values) if values is not None else None
+    kout = np.empty_like(keys)
+    vout = np.empty_like(vals) if vals is not None else None
+    rc = lib().ms_oracle_sort(_ptr(keys), _ptr(vals), _ptr(kout), _ptr(vout), keys.size, digit_bits)
+    if rc != 0:
+        raise ValueError(f"oracle sort failed ({rc})")
+    return kout, vout
+
+
+def histogram(keys, num_buckets: int, kind: int = RANGE, param: int = 0) -> np.ndarray:
+    keys = _u32(keys)
+    h = np.zeros(num_buckets, dtype=np.uint32)
+    lib().ms_oracle_histogram(kind, num_buckets, param, _ptr(keys), keys.size, _ptr(h))
+    return h
+
+
+def histogram_even_f32(samples, num_bins: int, lower: float, inv_delta: float) -> np.ndarray:
+    x = np.ascontiguousarray(samples, dtype=np.float32)
+    h = np.zeros(num_bins, dtype=np.int32)
+    lib().ms_oracle_histogram_even_f32(_ptr(x), x.size, _ptr(h), num_bins,
+                                       ctypes.c_float(lower), ctypes.c_float(inv_delta))
+    return h
+
+
+def compaction(keys, kind: int = LOWBITS, param: int = 1, values=None):
+    keys = _u32(keys)
+    vals = _u32(values) if values is not None else None
+    kout = np.empty_like(keys)
+    vout = np.empty_like(vals) if vals is not None else None
+    c = lib().ms_oracle_compaction(kind, param, _ptr(keys), _ptr(vals), _ptr(kout), _ptr(vout), keys.size)
+    return int(c), kout, vout
+
+
+def tile_histograms(keys, num_buckets: int, tile: int, kind: int = RANGE, param: int = 0) -> np.ndarray:
+    keys = _u32(keys)
+    nt = (keys.size + tile - 1) // tile
+    H = np.zeros(num_buckets * max(nt, 1), dtype=np.uint32)
+    lib().ms_oracle_tile_histograms(kind, num_buckets, param, _ptr(keys), keys.size, tile, _ptr(H))
+    return H[: num_buckets * nt]

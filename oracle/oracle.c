@@ -1,0 +1,251 @@
+/*
+ * CPU oracle for the multisplit hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * (or, for cpu_baseline, as the timed CPU restatement).  The product path in
+ * gpumultisplit_amd/ never links, loads or calls it.
+ *
+ * This is a clean-room, single-threaded restatement of the reference's CPU
+ * algorithm, written from its description (count -> exclusive scan -> stable
+ * scatter), not copied.  Every exported function cites the reference lines it
+ * follows (paths relative to the reference repository root).
+ *
+ * How each function is pinned (checks live in tests/test_oracle.py):
+ *   ms_oracle_bucket_ids     Python integer arithmetic for every functor kind,
+ *                            incl. edge keys 0, 2^32-1, delta boundaries.
+ *   ms_oracle_multisplit     numpy stable argsort of the bucket ids (a merge /
+ *                            radix sort, independent of counting), the closed
+ *                            form pos(i) = sum_{c<b_i} count(c) + #{j<i: b_j=b_i}
+ *                            evaluated by brute force on tiny n, special cases
+ *                            (single bucket => identity, pre-sorted => identity).
+ *   ms_oracle_sort           numpy sort / stable argsort of the keys.
+ *   ms_oracle_histogram      numpy bincount.
+ *   ms_oracle_histogram_even numpy float32 arithmetic + bincount.
+ *   ms_oracle_compaction     Python list partition; count = partition point.
+ * The reference itself ships no golden vectors or known-answer tests; the
+ * fixtures under tests/golden/ are produced by this oracle and cross-checked by
+ * the independent checks above when they are generated and again in tests.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* Bucket-identifier kinds.  Same numbering as include/gpumultisplit.h. */
+enum {
+    MS_OR_RANGE = 0,    /* floor(key * B / 2^32)  (range identifier)            */
+    MS_OR_DELTA = 1,    /* key / delta     (delta_func, src/main/main_multisplit.cu:41-47) */
+    MS_OR_DIGIT = 2,    /* (key >> shift) & (B-1) (ms_bucket_sort_Nbit,
+                           src/api/multisplit_sort.cuh:63-98)                  */
+    MS_OR_IDENTITY = 3, /* key             (identity_bucket, main_multisplit.cu:34-39) */
+    MS_OR_LOWBITS = 4   /* key & ((1<<flag)-1) (test_bucket<flag>,
+                           src/main/main_compaction.cu:27-32)                 */
+};
+
+/* One bucket id.  Ids >= B are undefined behaviour in the reference; the
+ * library defines them as clamped into bucket B-1, and the oracle states the
+ * same rule so that the library's documented behaviour can be checked. */
+static inline uint32_t bucket_of(uint32_t kind, uint32_t B, uint32_t param,
+                                 uint32_t key)
+{
+    uint32_t b;
+    switch (kind) {
+    case MS_OR_RANGE:    b = (uint32_t)(((uint64_t)key * (uint64_t)B) >> 32); break;
+    case MS_OR_DELTA:    b = key / param; break;
+    case MS_OR_DIGIT:    b = (key >> param) & (B - 1u); break;
+    case MS_OR_IDENTITY: b = key; break;
+    case MS_OR_LOWBITS:  b = key & ((param >= 32u) ? 0xFFFFFFFFu : ((1u << param) - 1u)); break;
+    default:             b = 0; break;
+    }
+    return b < B ? b : B - 1u;
+}
+
+void ms_oracle_bucket_ids(uint32_t kind, uint32_t B, uint32_t param,
+                          const uint32_t *keys, uint64_t n, uint32_t *ids)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        ids[i] = bucket_of(kind, B, param, keys[i]);
+}
+
+/*
+ * Stable multisplit, key-only (vin/vout NULL) or key-value.
+ * Follows cpu_multisplit_general (src/cpu_functions.h:155-220) and
+ * cpu_multisplit_pairs_general (src/cpu_functions.h:222-288):
+ *   1. per-bucket counts           (cpu_functions.h:167-171, :234-238)
+ *   2. exclusive scan of counts    (cpu_functions.h:179-181, :246-248)
+ *   3. stable scatter in input order with a running cursor per bucket
+ *                                  (cpu_functions.h:191-200, :258-267)
+ * offsets (optional, B entries) receives the exclusive scan, i.e. the global
+ * start of every bucket; this is what multisplit_key_* returns through its
+ * optional host array (src/api/multisplit.cuh:144-149, :261-266).
+ * Returns 0, or -1 on allocation failure.
+ */
+int ms_oracle_multisplit(uint32_t kind, uint32_t B, uint32_t param,
+                         const uint32_t *kin, const uint32_t *vin,
+                         uint32_t *kout, uint32_t *vout, uint64_t n,
+                         uint32_t *offsets)
+{
+    uint64_t *count = (uint64_t *)calloc(B, sizeof(uint64_t));
+    uint64_t *cursor = (uint64_t *)calloc(B, sizeof(uint64_t));
+    if (!count || !cursor) { free(count); free(cursor); return -1; }
+
+    for (uint64_t i = 0; i < n; ++i)
+        count[bucket_of(kind, B, param, kin[i])]++;
+
+    uint64_t run = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+        cursor[b] = run;
+        run += count[b];
+    }
+    if (offsets)
+        for (uint32_t b = 0; b < B; ++b) offsets[b] = (uint32_t)cursor[b];
+
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t b = bucket_of(kind, B, param, kin[i]);
+        uint64_t dst = cursor[b]++;
+        kout[dst] = kin[i];
+        if (vin && vout) vout[dst] = vin[i];
+    }
+    free(count);
+    free(cursor);
+    return 0;
+}
+
+/*
+ * Digit schedule of the multisplit radix sorts, as (shift, bits) pairs, least
+ * significant digit first:
+ *   8-bit: 4 x 8 bits at 0, 8, 16, 24            (multisplit_sort.cuh:529-559)
+ *   7-bit: 4 bits at 0, then 7 bits at 4, 11, 18, 25 (:320-364)
+ *   6-bit: 4 bits at 0, 4, then 6 bits at 8, 14, 20, 26 (:479-527)
+ *   5-bit: 5 bits at 0, 5, 10, 15, 20, 25, then 2 bits at 30 (:423-477)
+ *   4-bit: 8 x 4 bits at 0, 4, ..., 28           (:366-421)
+ * Key-value variants use the same schedules (multisplit_sort_pairs.cuh:21-260).
+ * Returns the number of passes, or 0 for an unsupported width.
+ */
+int ms_oracle_sort_schedule(uint32_t digit_bits, uint32_t *shifts, uint32_t *bits)
+{
+    int p = 0;
+    switch (digit_bits) {
+    case 8:
+        for (uint32_t s = 0; s < 32; s += 8) { shifts[p] = s; bits[p++] = 8; }
+        break;
+    case 7:
+        shifts[p] = 0; bits[p++] = 4;
+        for (uint32_t s = 4; s < 32; s += 7) { shifts[p] = s; bits[p++] = 7; }
+        break;
+    case 6:
+        shifts[p] = 0; bits[p++] = 4;
+        shifts[p] = 4; bits[p++] = 4;
+        for (uint32_t s = 8; s < 32; s += 6) { shifts[p] = s; bits[p++] = 6; }
+        break;
+    case 5:
+        for (uint32_t s = 0; s < 30; s += 5) { shifts[p] = s; bits[p++] = 5; }
+        shifts[p] = 30; bits[p++] = 2;
+        break;
+    case 4:
+        for (uint32_t s = 0; s < 32; s += 4) { shifts[p] = s; bits[p++] = 4; }
+        break;
+    default:
+        return 0;
+    }
+    return p;
+}
+
+/*
+ * LSD radix sort built from repeated stable multisplits on digits
+ * (multisplit_sort.cuh:320-559, multisplit_sort_pairs.cuh:21-260).  The input
+ * is never written; the result lands in kout/vout.  Returns 0 on success.
+ */
+int ms_oracle_sort(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                   uint32_t *vout, uint64_t n, uint32_t digit_bits)
+{
+    uint32_t shifts[8], bits[8];
+    int passes = ms_oracle_sort_schedule(digit_bits, shifts, bits);
+    if (passes == 0) return -2;
+    int kv = (vin != NULL && vout != NULL);
+    uint32_t *ka = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint32_t *kb = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+    uint32_t *va = kv ? (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t)) : NULL;
+    uint32_t *vb = kv ? (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t)) : NULL;
+    if (!ka || !kb || (kv && (!va || !vb))) {
+        free(ka); free(kb); free(va); free(vb);
+        return -1;
+    }
+    memcpy(ka, kin, n * sizeof(uint32_t));
+    if (kv) memcpy(va, vin, n * sizeof(uint32_t));
+    for (int p = 0; p < passes; ++p) {
+        ms_oracle_multisplit(MS_OR_DIGIT, 1u << bits[p], shifts[p], ka,
+                             kv ? va : NULL, kb, kv ? vb : NULL, n, NULL);
+        uint32_t *t = ka; ka = kb; kb = t;
+        if (kv) { t = va; va = vb; vb = t; }
+    }
+    memcpy(kout, ka, n * sizeof(uint32_t));
+    if (kv) memcpy(vout, va, n * sizeof(uint32_t));
+    free(ka); free(kb); free(va); free(vb);
+    return 0;
+}
+
+/*
+ * Histogram of bucket ids: hist[b] += #{i : f(key_i) = b}.  Accumulates, as
+ * the reference's histogram kernels atomically add into a caller-zeroed array
+ * (src/kernels/histogram/multisplit2_histograms.cuh:36-392).
+ */
+void ms_oracle_histogram(uint32_t kind, uint32_t B, uint32_t param,
+                         const uint32_t *keys, uint64_t n, uint32_t *hist)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        hist[bucket_of(kind, B, param, keys[i])]++;
+}
+
+/*
+ * Even-bin histogram of float samples: bin = (uint32)((x - lower) * inv_delta)
+ * evaluated in float32, as multisplit2_histogram_even_* computes it
+ * (multisplit2_histograms.cuh:60, :330).  Samples whose bin falls outside
+ * [0, nbins) are ignored (the reference does not guard; CUB's HistogramEven,
+ * which its driver compares against, ignores them).  Accumulates into hist.
+ */
+void ms_oracle_histogram_even_f32(const float *x, uint64_t n, int32_t *hist,
+                                  uint32_t nbins, float lower, float inv_delta)
+{
+    for (uint64_t i = 0; i < n; ++i) {
+        float d = x[i] - lower;
+        if (!(d >= 0.0f)) continue;           /* negative or NaN */
+        float s = d * inv_delta;
+        if (!(s < (float)nbins)) continue;
+        uint32_t b = (uint32_t)s;
+        if (b < nbins) hist[b]++;
+    }
+}
+
+/*
+ * Stable 2-bucket multisplit (bucket 0 first) that returns the bucket-0 count,
+ * as compaction_key_only / compaction_key_value do
+ * (src/kernels/compaction/multisplit2_compaction.cuh:525-573, return value
+ * read at :541-542).  Returns the count, or UINT64_MAX on error.
+ */
+uint64_t ms_oracle_compaction(uint32_t kind, uint32_t param, const uint32_t *kin,
+                              const uint32_t *vin, uint32_t *kout,
+                              uint32_t *vout, uint64_t n)
+{
+    uint32_t offsets[2];
+    if (ms_oracle_multisplit(kind, 2u, param, kin, vin, kout, vout, n, offsets) != 0)
+        return UINT64_MAX;
+    uint64_t zeros = 0;
+    for (uint64_t i = 0; i < n; ++i)
+        zeros += (bucket_of(kind, 2u, param, kin[i]) == 0u);
+    return zeros;
+}
+
+/* Per-tile bucket counts in the library's bucket-major layout
+ * H[b * num_tiles + t] = #{ keys of tile t in bucket b }, tiles of `tile`
+ * consecutive keys.  Same quantity the reference's unused checker
+ * cpu_prescan_test computes (src/cpu_functions.h:55-103). */
+void ms_oracle_tile_histograms(uint32_t kind, uint32_t B, uint32_t param,
+                               const uint32_t *keys, uint64_t n, uint32_t tile,
+                               uint32_t *H)
+{
+    uint64_t nt = (n + tile - 1) / tile;
+    memset(H, 0, (size_t)(B * nt) * sizeof(uint32_t));
+    for (uint64_t i = 0; i < n; ++i)
+        H[(uint64_t)bucket_of(kind, B, param, keys[i]) * nt + i / tile]++;
+}

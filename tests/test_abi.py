@@ -1,0 +1,133 @@
+"""C-ABI library checks that need no GPU: the library loads, exports exactly the
+symbols include/gpumultisplit.h declares, and its host-side logic (bucket-id
+arithmetic, tile policy, argument validation before any HIP call) is right."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import gpumultisplit_amd as gms
+import oracle
+from gpumultisplit_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gpumultisplit.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gms_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_loads_and_reports_abi_version():
+    L = gms.load_library()
+    assert L.gms_abi_version() == 1
+
+
+def test_exports_match_header():
+    declared = header_functions()
+    out = subprocess.run(["nm", "-D", "--defined-only", gms.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = sorted({ln.split()[-1] for ln in out.splitlines() if " T gms_" in ln})
+    assert declared == exported
+    assert sorted(gms.EXPORTED_SYMBOLS) == declared
+    L = gms.load_library()
+    for name in declared:
+        assert getattr(L, name) is not None
+
+
+def test_status_strings():
+    L = gms.load_library()
+    for code in range(9):
+        assert L.gms_status_string(code)
+    assert L.gms_status_string(0).decode() == "success"
+
+
+@pytest.mark.parametrize("delta", [1, 2, 3, 7, 10, 1000, 32768, 1048576, (1 << 25) // 32,
+                                   0x7FFFFFFF, 0x80000000, 0x80000001, 0xFFFFFFFE, 0xFFFFFFFF,
+                                   12345679, 641, 6700417])
+def test_delta_magic_division_is_exact(delta):
+    # The kernels divide by multiplying with a precomputed reciprocal; the host
+    # entry point runs the same code.  Check against exact integer division on
+    # keys around every multiple of delta and at the ends of the key range.
+    g = W.rng(delta % 1000 + 1)
+    keys = [0, 1, 2, 0xFFFFFFFF, 0xFFFFFFFE, 0x80000000, 0x7FFFFFFF]
+    for m in list(range(0, 64)) + list(g.integers(0, (1 << 32) // delta + 1, 64)):
+        for o in (-1, 0, 1):
+            k = int(m) * delta + o
+            if 0 <= k < (1 << 32):
+                keys.append(k)
+    keys = np.array(keys + list(W.uniform_keys(20000, seed=delta % 977)), dtype=np.uint64).astype(np.uint32)
+    ids = gms.bucket_ids_host(keys, 256, gms.delta_func(delta))
+    want = np.minimum(keys.astype(np.uint64) // delta, 255).astype(np.uint32)
+    np.testing.assert_array_equal(ids, want)
+
+
+@pytest.mark.parametrize("B", [1, 2, 3, 7, 8, 100, 255, 256])
+def test_range_and_other_ids_match_oracle(B):
+    keys = np.concatenate([W.uniform_keys(5000, seed=B),
+                           np.array([0, 1, 0xFFFFFFFF, 0x80000000], dtype=np.uint32)])
+    np.testing.assert_array_equal(gms.bucket_ids_host(keys, B, gms.range_bucket(B)),
+                                  oracle.bucket_ids(keys, B, oracle.RANGE))
+    np.testing.assert_array_equal(gms.bucket_ids_host(keys, B, gms.identity_bucket()),
+                                  oracle.bucket_ids(keys, B, oracle.IDENTITY))
+    if B & (B - 1) == 0 and B > 1:
+        bits = B.bit_length() - 1
+        digit = gms._digit_functor(bits)
+        for shift in (0, 5, 32 - bits):
+            np.testing.assert_array_equal(gms.bucket_ids_host(keys, B, digit(shift)),
+                                          oracle.bucket_ids(keys, B, oracle.DIGIT, shift))
+        np.testing.assert_array_equal(gms.bucket_ids_host(keys, B, gms.test_bucket(bits)),
+                                      oracle.bucket_ids(keys, B, oracle.LOWBITS, bits))
+
+
+def test_digit_functors_match_oracle():
+    keys = W.uniform_keys(4000, seed=44)
+    for bits, cls in ((2, gms.ms_bucket_sort_2bit), (4, gms.ms_bucket_sort_4bit),
+                      (5, gms.ms_bucket_sort_5bit), (6, gms.ms_bucket_sort_6bit),
+                      (7, gms.ms_bucket_sort_7bit), (8, gms.ms_bucket_sort_8bit)):
+        for shift in (0, 3, 32 - bits):
+            f = cls(shift)
+            got = gms.bucket_ids_host(keys, 1 << bits, f)
+            np.testing.assert_array_equal(got, oracle.bucket_ids(keys, 1 << bits, oracle.DIGIT, shift))
+            assert all(int(got[i]) == f(int(keys[i])) for i in range(50))
+
+
+def test_descriptor_validation_on_host():
+    keys = np.zeros(4, dtype=np.uint32)
+    with pytest.raises(gms.GmsError) as e:
+        gms.bucket_ids_host(keys, 0, gms.range_bucket(0))
+    assert e.value.status == 2
+    with pytest.raises(gms.GmsError):
+        gms.bucket_ids_host(keys, 257, gms.range_bucket(257))
+    with pytest.raises(gms.GmsError):
+        gms.bucket_ids_host(keys, 6, gms.ms_bucket_sort_4bit(0))      # 16 digit values > 6 buckets
+    np.testing.assert_array_equal(                                    # 4 digit values fit 6 buckets
+        gms.bucket_ids_host(np.arange(8, dtype=np.uint32), 6, gms.ms_bucket_sort_2bit(1)),
+        np.array([0, 0, 1, 1, 2, 2, 3, 3], dtype=np.uint32))
+    with pytest.raises(gms.GmsError):
+        gms.bucket_ids_host(keys, 2, gms.test_bucket(2))              # 4 ids > 2 buckets
+    with pytest.raises(ValueError):
+        gms.delta_func(0)
+
+
+def test_tile_policy():
+    for B in (2, 8, 32, 64, 256):
+        t_ko = gms.subproblem_size_bms_key_only(B)
+        t_kv = gms.subproblem_size_bms_key_value(B)
+        assert t_ko % 256 == 0 and t_kv % 256 == 0 and 1024 <= t_kv <= 32768
+    assert gms.lib().gms_subproblem_size(0, 1) == 0
+    assert gms.lib().gms_subproblem_size(257, 1) == 0
+
+
+def test_context_creation_checks_bucket_count():
+    with pytest.raises(gms.GmsError) as e:
+        gms.multisplit_context(0)
+    assert e.value.status == 2
+    with pytest.raises(gms.GmsError):
+        gms.multisplit_context(300)
+    ctx = gms.multisplit_context(256)       # host-only object, no HIP call yet
+    assert ctx.num_buckets == 256

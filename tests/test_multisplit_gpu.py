@@ -1,0 +1,310 @@
+"""Parity of the HIP multisplit with the CPU oracle, through the C ABI.
+
+Mirrors the reference's driver checks (src/main/main_multisplit.cu:172-192,
+:239-267: run the GPU path, run cpu_multisplit_*_general, compare element by
+element) and closes its blind spots: values are the original index (so
+stability is observable), n is ragged around tile boundaries, skewed bucket
+distributions are exercised, and bucket offsets are compared.  At the full
+sizes (2^25, 2^27) results are checked through properties that fix a stable
+multisplit completely: values are a permutation of 0..n-1, keys equal the
+input gathered by values, bucket ids are non-decreasing, values increase
+inside each bucket, and offsets are the exclusive scan of the bucket counts.
+"""
+import numpy as np
+import pytest
+
+import gpumultisplit_amd as gms
+import oracle
+from gpumultisplit_amd import workloads as W
+from gpu_util import banner, empty_u32, filled_u32, from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def run_ms(dev, keys, B, functor, kv=True, capacity=None, values=None, want_offsets=True):
+    n = keys.size
+    ctx = gms.multisplit_context(B)
+    (gms.multisplit_allocate_key_value if kv else gms.multisplit_allocate_key_only)(
+        capacity if capacity is not None else n, ctx)
+    kin = to_dev(keys, dev)
+    kout = filled_u32(n, dev)
+    offs = np.zeros(B, dtype=np.uint32) if want_offsets else None
+    if kv:
+        vals = np.arange(n, dtype=np.uint32) if values is None else values
+        vin = to_dev(vals, dev)
+        vout = filled_u32(n, dev)
+        gms.multisplit_key_value(kin, vin, kout, vout, n, ctx, functor, offs)
+        res = from_dev(kout)[:n], from_dev(vout)[:n], offs
+    else:
+        gms.multisplit_key_only(kin, kout, n, ctx, functor, offs)
+        res = from_dev(kout)[:n], None, offs
+    gms.multisplit_release_memory(ctx)
+    return res
+
+
+def oracle_ms(keys, B, kind, param=0, kv=True, values=None):
+    vals = (np.arange(keys.size, dtype=np.uint32) if values is None else values) if kv else None
+    return oracle.multisplit(keys, B, kind, param, vals)
+
+
+def check_equal(got, want, kv):
+    np.testing.assert_array_equal(got[0], want[0])
+    if kv:
+        np.testing.assert_array_equal(got[1], want[1])
+    if got[2] is not None:
+        np.testing.assert_array_equal(got[2], want[2])
+
+
+TILE_EDGE_SIZES = [0, 1, 2, 63, 64, 65, 1000, 4095, 4096, 4097, 3 * 4096 + 1, 8192, 100_003]
+
+
+@pytest.mark.parametrize("kv", [False, True])
+@pytest.mark.parametrize("B", [1, 2, 3, 4, 5, 8, 16, 17, 32, 33, 64, 100, 128, 129, 255, 256])
+def test_range_identifier_all_bucket_counts_ragged(dev, B, kv):
+    for n in TILE_EDGE_SIZES:
+        banner(f"range B={B} kv={kv} n={n}")
+        keys = W.uniform_keys(n, seed=B * 131 + n)
+        got = run_ms(dev, keys, B, gms.range_bucket(B), kv)
+        check_equal(got, oracle_ms(keys, B, oracle.RANGE, kv=kv), kv)
+
+
+@pytest.mark.parametrize("kv", [False, True])
+@pytest.mark.parametrize("dist", ["uniform", "zipf", "onehot90"])
+@pytest.mark.parametrize("B", [2, 8, 32, 64, 256])
+def test_sweep_distributions_2e20(dev, dist, B, kv):
+    n = 1 << 20
+    banner(f"sweep {dist} B={B} kv={kv} n={n}")
+    keys = W.keys_for(dist, n, B, seed=7 * B + len(dist))
+    got = run_ms(dev, keys, B, gms.range_bucket(B), kv)
+    check_equal(got, oracle_ms(keys, B, oracle.RANGE, kv=kv), kv)
+
+
+def test_config0_keyonly_2e20_8_buckets(dev):
+    # 2^20 uint32 keys uniform over [0, 2^32), 8 buckets by range identifier,
+    # key-only, bit-exact against the CPU path.
+    keys = W.uniform_keys(1 << 20, seed=20)
+    got = run_ms(dev, keys, 8, gms.range_bucket(8), kv=False)
+    check_equal(got, oracle_ms(keys, 8, oracle.RANGE, kv=False), False)
+
+
+@pytest.mark.parametrize("kv", [False, True])
+def test_delta_identifier_like_reference_driver(dev, kv):
+    # main_multisplit.cu:98-108: B = 32, delta = ceil(n / B), keys in [b*delta, (b+1)*delta)
+    for n in (1 << 16, (1 << 20) + 7):
+        B = 32
+        delta = (n + B - 1) // B
+        keys = W.delta_keys(n, B, delta, seed=n)
+        banner(f"delta n={n} kv={kv}")
+        got = run_ms(dev, keys, B, gms.delta_func(delta), kv)
+        check_equal(got, oracle_ms(keys, B, oracle.DELTA, delta, kv=kv), kv)
+
+
+@pytest.mark.parametrize("B", [2, 7, 64, 256])
+def test_delta_identifier_odd_deltas_and_clamping(dev, B):
+    keys = W.uniform_keys(50_001, seed=B)
+    for delta in (1, 3, 1000, 0x01000001, 0xFFFFFFFF):
+        banner(f"delta B={B} delta={delta}")
+        got = run_ms(dev, keys, B, gms.delta_func(delta), True)
+        check_equal(got, oracle_ms(keys, B, oracle.DELTA, delta), True)
+
+
+@pytest.mark.parametrize("B", [2, 16, 256])
+def test_identity_identifier(dev, B):
+    # main_wms.cu drives identity_bucket with keys = shuffled i % B
+    n = 40_000
+    keys = (W.permutation_keys(n, seed=B) % B).astype(np.uint32)
+    got = run_ms(dev, keys, B, gms.identity_bucket(), True)
+    check_equal(got, oracle_ms(keys, B, oracle.IDENTITY), True)
+    # ids >= B are clamped into bucket B-1 (undefined in the reference)
+    keys2 = W.uniform_keys(n, seed=B + 1) % np.uint32(2 * B)
+    got = run_ms(dev, keys2, B, gms.identity_bucket(), True)
+    check_equal(got, oracle_ms(keys2, B, oracle.IDENTITY), True)
+
+
+@pytest.mark.parametrize("bits,shift", [(8, 0), (8, 24), (7, 4), (7, 25), (4, 28), (2, 30), (6, 13)])
+def test_digit_identifier(dev, bits, shift):
+    keys = W.uniform_keys(70_000, seed=bits * 32 + shift)
+    B = 1 << bits
+    got = run_ms(dev, keys, B, gms._digit_functor(bits)(shift), True)
+    check_equal(got, oracle_ms(keys, B, oracle.DIGIT, shift), True)
+
+
+def test_lowbits_identifier(dev):
+    keys = W.permutation_keys(33_333, seed=5)
+    for flag, B in ((1, 2), (3, 8), (2, 8)):
+        got = run_ms(dev, keys, B, gms.test_bucket(flag), False)
+        check_equal(got, oracle_ms(keys, B, oracle.LOWBITS, flag, kv=False), False)
+
+
+def test_random_payload_values(dev):
+    keys = W.uniform_keys(123_457, seed=9)
+    vals = W.uniform_keys(123_457, seed=10)
+    got = run_ms(dev, keys, 64, gms.range_bucket(64), True, values=vals)
+    check_equal(got, oracle_ms(keys, 64, oracle.RANGE, values=vals), True)
+
+
+def test_call_smaller_than_capacity_and_reuse(dev):
+    B = 256
+    ctx = gms.multisplit_context(B)
+    gms.multisplit_allocate_key_value(1 << 18, ctx)
+    for n in (1 << 18, 5000, 1, 0, 77_777):
+        banner(f"reuse n={n}")
+        keys = W.uniform_keys(n, seed=n + 3)
+        vals = np.arange(n, dtype=np.uint32)
+        kin, vin = to_dev(keys, dev), to_dev(vals, dev)
+        kout, vout = filled_u32(n, dev), filled_u32(n, dev)
+        offs = np.full(B, 7, dtype=np.uint32)
+        gms.multisplit_key_value(kin, vin, kout, vout, n, ctx, gms.range_bucket(B), offs)
+        wk, wv, wo = oracle_ms(keys, B, oracle.RANGE)
+        np.testing.assert_array_equal(from_dev(kout)[:n], wk)
+        np.testing.assert_array_equal(from_dev(vout)[:n], wv)
+        np.testing.assert_array_equal(offs, wo)
+    gms.multisplit_release_memory(ctx)
+
+
+def test_unaligned_pointers_take_scalar_path(dev):
+    import torch
+    n, B = 20_000, 128
+    keys = W.uniform_keys(n + 3, seed=12)
+    vals = np.arange(n + 3, dtype=np.uint32)
+    kin_full, vin_full = to_dev(keys, dev), to_dev(vals, dev)
+    kout_full = torch.zeros(n + 3, dtype=torch.int32, device=dev)
+    vout_full = torch.zeros(n + 3, dtype=torch.int32, device=dev)
+    ctx = gms.multisplit_context(B)
+    gms.multisplit_allocate_key_value(n, ctx)
+    gms.multisplit_key_value(kin_full[1:], vin_full[3:], kout_full[2:], vout_full[1:], n, ctx,
+                             gms.range_bucket(B))
+    wk, wv, _ = oracle.multisplit(keys[1:n + 1], B, oracle.RANGE, 0, vals[3:n + 3])
+    np.testing.assert_array_equal(from_dev(kout_full)[2:n + 2], wk)
+    np.testing.assert_array_equal(from_dev(vout_full)[1:n + 1], wv)
+    gms.multisplit_release_memory(ctx)
+
+
+def test_device_offsets_and_determinism(dev):
+    import torch
+    n, B = 300_000, 256
+    keys = W.keys_for("zipf", n, B, seed=4)
+    ctx = gms.multisplit_context(B)
+    gms.multisplit_allocate_key_value(n, ctx)
+    kin, vin = to_dev(keys, dev), to_dev(np.arange(n, dtype=np.uint32), dev)
+    outs = []
+    for rep in range(3):
+        kout, vout = filled_u32(n, dev), filled_u32(n, dev)
+        d_off = torch.zeros(B, dtype=torch.int32, device=dev)
+        gms.multisplit(kin, kout, n, ctx, gms.range_bucket(B), vin, vout, d_off)
+        torch.cuda.synchronize()
+        outs.append((from_dev(kout), from_dev(vout), from_dev(d_off)))
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            np.testing.assert_array_equal(a, b)
+    wk, wv, wo = oracle_ms(keys, B, oracle.RANGE)
+    np.testing.assert_array_equal(outs[0][0], wk)
+    np.testing.assert_array_equal(outs[0][1], wv)
+    np.testing.assert_array_equal(outs[0][2], wo)
+    gms.multisplit_release_memory(ctx)
+
+
+def test_prescan_matrix_stage(dev):
+    import torch
+    for B, n in ((256, 100_000), (8, 50_001), (3, 9000)):
+        ctx = gms.multisplit_context(B)
+        gms.multisplit_allocate_key_value(n, ctx)
+        keys = W.uniform_keys(n, seed=B)
+        tile = gms.subproblem_size_bms_key_value(B)
+        nt = (n + tile - 1) // tile
+        mat = torch.zeros(B * nt, dtype=torch.int32, device=dev)
+        gms.prescan_matrix(to_dev(keys, dev), n, ctx, gms.range_bucket(B), mat, key_value=True)
+        np.testing.assert_array_equal(from_dev(mat), oracle.tile_histograms(keys, B, tile))
+        gms.multisplit_release_memory(ctx)
+
+
+@pytest.mark.parametrize("m", [1, 5, 4095, 4096, 4097, 1 << 20, (1 << 22) + 3])
+def test_exclusive_scan_stage(dev, m):
+    import torch
+    ctx = gms.multisplit_context(256)
+    gms.multisplit_allocate_key_value(max(m, 4096) * 2, ctx)   # scratch for m partials
+    x = (W.uniform_keys(m, seed=m) % 1000).astype(np.uint32)
+    d_in = to_dev(x, dev)
+    d_out = torch.zeros(m, dtype=torch.int32, device=dev)
+    gms.exclusive_scan(d_in, d_out, m, ctx)
+    want = (np.concatenate([[0], np.cumsum(x.astype(np.uint64))[:-1]]) & 0xFFFFFFFF).astype(np.uint32)
+    np.testing.assert_array_equal(from_dev(d_out), want)
+    gms.exclusive_scan(d_in, d_in, m, ctx)                    # in place
+    np.testing.assert_array_equal(from_dev(d_in), want)
+    gms.multisplit_release_memory(ctx)
+
+
+def test_error_paths(dev):
+    import torch
+    ctx = gms.multisplit_context(16)
+    k = torch.zeros(100, dtype=torch.int32, device=dev)
+    o = torch.zeros(100, dtype=torch.int32, device=dev)
+    with pytest.raises(gms.GmsError) as e:
+        gms.multisplit_key_only(k, o, 100, ctx, gms.range_bucket(16))
+    assert e.value.status == 4                                   # not allocated
+    gms.multisplit_allocate_key_only(50, ctx)
+    with pytest.raises(gms.GmsError) as e:
+        gms.multisplit_key_only(k, o, 60, ctx, gms.range_bucket(16))
+    assert e.value.status == 3                                   # capacity
+    with pytest.raises(gms.GmsError) as e:
+        gms.multisplit_key_only(k, k, 50, ctx, gms.range_bucket(16))
+    assert e.value.status == 7                                   # in/out alias
+    with pytest.raises(gms.GmsError) as e:
+        gms.multisplit_key_only(k[:50], k[25:75], 50, ctx, gms.range_bucket(16))
+    assert e.value.status == 7
+    with pytest.raises(gms.GmsError) as e:
+        gms.multisplit_key_only(k, o, 50, ctx, gms.ms_bucket_sort_2bit(0).__class__(40))
+    assert e.value.status == 1                                   # shift >= 32
+    gms.multisplit_release_memory(ctx)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("n,dist", [(1 << 27, "uniform"), (1 << 25, "uniform"), (1 << 25, "zipf"),
+                                    (1 << 25, "onehot90")])
+def test_full_size_256_kv_properties(dev, n, dist):
+    """2^25 / 2^27 keys, 256 buckets, key-value: complete property check on the GPU."""
+    import torch
+    B = 256
+    banner(f"full-size {dist} n={n}")
+    if dist == "uniform":
+        g = torch.Generator(device=dev)
+        g.manual_seed(n)
+        kin = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=dev, generator=g)
+    else:
+        kin = to_dev(W.keys_for(dist, n, B, seed=3), dev)
+    vin = torch.arange(n, dtype=torch.int32, device=dev)
+    kout = torch.empty_like(kin)
+    vout = torch.empty_like(vin)
+    d_off = torch.zeros(B, dtype=torch.int32, device=dev)
+    ctx = gms.multisplit_context(B)
+    gms.multisplit_allocate_key_value(n, ctx)
+    gms.multisplit(kin, kout, n, ctx, gms.range_bucket(B), vin, vout, d_off)
+    torch.cuda.synchronize()
+    gms.multisplit_release_memory(ctx)
+    ku = kin.to(torch.int64) & 0xFFFFFFFF
+    bid_in = (ku * B) >> 32
+    ko = kout.to(torch.int64) & 0xFFFFFFFF
+    bid_out = (ko * B) >> 32
+    vo = vout.to(torch.int64)
+    # values are a permutation of 0..n-1
+    seen = torch.zeros(n, dtype=torch.int32, device=dev)
+    seen.index_add_(0, vo, torch.ones(n, dtype=torch.int32, device=dev))
+    assert bool((seen == 1).all())
+    # keys travel with their values
+    assert bool((kout == kin[vo]).all())
+    # bucket ids non-decreasing; values increasing inside a bucket (stability)
+    assert bool((bid_out[1:] >= bid_out[:-1]).all())
+    same = bid_out[1:] == bid_out[:-1]
+    assert bool((vo[1:][same] > vo[:-1][same]).all())
+    # offsets = exclusive scan of bucket counts
+    cnt = torch.bincount(bid_in, minlength=B)
+    want = torch.cumsum(cnt, 0) - cnt
+    assert bool((d_off.to(torch.int64) == want).all())
+
+
+@pytest.mark.slow
+def test_2e25_kv_256_against_oracle(dev):
+    n, B = 1 << 25, 256
+    keys = W.uniform_keys(n, seed=25)
+    got = run_ms(dev, keys, B, gms.range_bucket(B), True)
+    check_equal(got, oracle_ms(keys, B, oracle.RANGE), True)
