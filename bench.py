@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-stage-events", action="store_true")
     p.add_argument("--cpu-reps", type=int, default=5)
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="run one plain copy of n u32 before warm-up (HBM counter calibration)")
     return p.parse_args()
 
 
@@ -118,6 +120,9 @@ def main():
         if st != 0:
             raise gms.GmsError(st, "gms_multisplit")
 
+    if args.calibrate_copy:
+        gms.copy_u32(kin, kout, n)
+        torch.cuda.synchronize()
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()

@@ -2,7 +2,21 @@
 // identifiers, wave64 scans on DPP, and the XCD-aware tile order.
 #pragma once
 
+#ifdef GMS_HOST_EMULATION
+// CPU build of the kernels for tests (tests/emu): same sources, emulated runtime.
+#include "hip_emu.hpp"
+#define GMS_UPDATE_DPP gms_emu::update_dpp
+#define GMS_READLANE gms_emu::readlane
+#define GMS_WAVE_SYNC() gms_emu::wave_sync()
+#else
 #include <hip/hip_runtime.h>
+#define GMS_UPDATE_DPP __builtin_amdgcn_update_dpp
+#define GMS_READLANE __builtin_amdgcn_readlane
+// Orders the LDS accesses of one wave before and after this point across its
+// lanes.  The lanes of a wave execute in lockstep, so this only stops the
+// compiler from moving memory operations across it; it emits no instruction.
+#define GMS_WAVE_SYNC() __builtin_amdgcn_wave_barrier()
+#endif
 #include <stdint.h>
 
 #define GMS_HD __host__ __device__ __forceinline__
@@ -72,18 +86,18 @@ struct IdentF {
 // inside a row, row_bcast:15 / row_bcast:31 carry them across rows.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
-    x += __builtin_amdgcn_update_dpp(0u, x, 0x111, 0xF, 0xF, false);  // row_shr:1
-    x += __builtin_amdgcn_update_dpp(0u, x, 0x112, 0xF, 0xF, false);  // row_shr:2
-    x += __builtin_amdgcn_update_dpp(0u, x, 0x114, 0xF, 0xF, false);  // row_shr:4
-    x += __builtin_amdgcn_update_dpp(0u, x, 0x118, 0xF, 0xF, false);  // row_shr:8
-    x += __builtin_amdgcn_update_dpp(0u, x, 0x142, 0xA, 0xF, false);  // row_bcast:15
-    x += __builtin_amdgcn_update_dpp(0u, x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    x += GMS_UPDATE_DPP(0u, x, 0x111, 0xF, 0xF, false);  // row_shr:1
+    x += GMS_UPDATE_DPP(0u, x, 0x112, 0xF, 0xF, false);  // row_shr:2
+    x += GMS_UPDATE_DPP(0u, x, 0x114, 0xF, 0xF, false);  // row_shr:4
+    x += GMS_UPDATE_DPP(0u, x, 0x118, 0xF, 0xF, false);  // row_shr:8
+    x += GMS_UPDATE_DPP(0u, x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    x += GMS_UPDATE_DPP(0u, x, 0x143, 0xC, 0xF, false);  // row_bcast:31
     return x;
 }
 
 // Value of lane 63 broadcast to the wave (the total after an inclusive scan).
 __device__ __forceinline__ uint32_t wave_last(uint32_t x) {
-    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+    return (uint32_t)GMS_READLANE((int)x, 63);
 }
 
 // Exclusive scan across a block of NW waves; `total` receives the block sum.

@@ -1,7 +1,6 @@
 // Host-side engine shared by the C-ABI entry points.
 #pragma once
 
-#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include <vector>

@@ -8,11 +8,16 @@
 //
 // Same decomposition as the reference (BMS prescan, CUB scan, BMS postscan;
 // src/api/multisplit.cuh:199-313).  The in-tile ranking differs: instead of
-// warp ballots per bucket bit, each thread owns a column of packed 16-bit
-// digit counters in LDS and ranks its K consecutive keys with ds_add_rtn, a
-// block-wide column scan turns the counters into positions, and digits wider
-// than LEVEL_BITS are ranked least-significant part first in up to three
-// stable LDS passes.
+// warp ballots per bucket bit, each thread owns a column of digit counters in
+// LDS and ranks its K consecutive keys with ds_add_rtn; a block-wide scan of
+// the counters in memory (digit-major) order turns them into positions.
+// Bucket ids wider than LEVEL_BITS are ranked least-significant part first in
+// up to three stable LDS passes.
+//
+// One workgroup per tile, tiles ordered so that every XCD works on one
+// contiguous range (xcd_tile).  Full tiles take a branch-free path whose loads
+// are all unconditional, so the key and value loads of a thread are in flight
+// together; only the ragged last tile takes guarded loads.
 #pragma once
 
 #include "gms_device.hpp"
@@ -20,16 +25,25 @@
 
 namespace gms {
 
-// K consecutive u32 of a tile into registers (thread t owns [t*K, t*K+K)).
-template <int K>
-__device__ __forceinline__ void load_blocked(const uint32_t* __restrict__ p, size_t base,
-                                             uint32_t nvalid, uint32_t (&x)[K], bool vec) {
+__device__ __forceinline__ uint32_t tile_valid(uint32_t n, uint32_t t, uint32_t tile) {
+    const size_t base = (size_t)t * tile;
+    return (uint32_t)(((size_t)n - base) < tile ? ((size_t)n - base) : tile);
+}
+
+// Tile loads.  Full tiles use unconditional loads (16-byte vectors when VEC),
+// so that a load issued one tile ahead is not waited for until its data is
+// used; only the ragged last tile of an input takes the guarded path.
+//
+// Blocked: thread t owns [t*K, t*K+K).
+template <int K, bool VEC>
+__device__ __forceinline__ void load_blocked_full(const uint32_t* __restrict__ p, size_t base,
+                                                  uint32_t (&x)[K]) {
     const uint32_t off = threadIdx.x * (uint32_t)K;
-    if (vec) {
+    if constexpr (VEC) {
         const uint4* q = reinterpret_cast<const uint4*>(p + base + off);
 #pragma unroll
         for (int i = 0; i < K / 4; ++i) {
-            uint4 v = q[i];
+            const uint4 v = q[i];
             x[4 * i + 0] = v.x;
             x[4 * i + 1] = v.y;
             x[4 * i + 2] = v.z;
@@ -37,77 +51,114 @@ __device__ __forceinline__ void load_blocked(const uint32_t* __restrict__ p, siz
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < K; ++k) x[k] = (off + (uint32_t)k < nvalid) ? p[base + off + k] : 0u;
+        for (int k = 0; k < K; ++k) x[k] = p[base + off + k];
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void load_blocked_guarded(const uint32_t* __restrict__ p, size_t base,
+                                                     uint32_t nvalid, uint32_t (&x)[K]) {
+    const uint32_t off = threadIdx.x * (uint32_t)K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) x[k] = (off + (uint32_t)k < nvalid) ? p[base + off + k] : 0u;
+}
+
+// Striped by 16 bytes: thread t owns [4(j*256+t), +4) for j < K/4.
+template <int K, bool VEC>
+__device__ __forceinline__ void load_striped4_full(const uint32_t* __restrict__ p, size_t base,
+                                                   uint32_t (&x)[K]) {
+#pragma unroll
+    for (int j = 0; j < K / 4; ++j) {
+        const uint32_t w4 = 4u * ((uint32_t)j * kThreads + threadIdx.x);
+        if constexpr (VEC) {
+            const uint4 v = *reinterpret_cast<const uint4*>(p + base + w4);
+            x[4 * j + 0] = v.x;
+            x[4 * j + 1] = v.y;
+            x[4 * j + 2] = v.z;
+            x[4 * j + 3] = v.w;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) x[4 * j + i] = p[base + w4 + i];
+        }
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void load_striped4_guarded(const uint32_t* __restrict__ p, size_t base,
+                                                      uint32_t nvalid, uint32_t (&x)[K]) {
+#pragma unroll
+    for (int j = 0; j < K / 4; ++j) {
+        const uint32_t w4 = 4u * ((uint32_t)j * kThreads + threadIdx.x);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[4 * j + i] = (w4 + i < nvalid) ? p[base + w4 + i] : 0u;
     }
 }
 
 // ---------------------------------------------------------------------------
 // Prescan: per-tile bucket counts.
 // ---------------------------------------------------------------------------
-template <int LOG_B, int K, class F>
-__global__ __launch_bounds__(kThreads) void ms_prescan_kernel(const uint32_t* __restrict__ kin,
-                                                              uint32_t n, uint32_t num_tiles,
-                                                              uint32_t* __restrict__ H, F f,
-                                                              int vec_ok) {
+// Counts of one tile into s_h (per-wave sub-histograms), then H[:, t].
+template <int LOG_B, int K, bool FULL, class F>
+__device__ __forceinline__ void prescan_tile(const uint32_t (&x)[K], uint32_t nvalid, uint32_t t,
+                                             uint32_t num_tiles, uint32_t* __restrict__ H, F f,
+                                             uint32_t* s_h) {
     constexpr int B = 1 << LOG_B;
-    constexpr uint32_t TILE = kThreads * K;
-    __shared__ uint32_t s_h[kWaves * B];
-
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
-    const uint32_t tile = xcd_tile(blockIdx.x, num_tiles);
-    const size_t base = (size_t)tile * TILE;
-    const uint32_t nvalid = (uint32_t)(((size_t)n - base) < TILE ? ((size_t)n - base) : TILE);
-    const bool vec = vec_ok && nvalid == TILE;
-
     if constexpr (LOG_B <= 3) {
         // Byte counters for up to 8 buckets in one 64-bit register (K <= 255).
         uint64_t pc = 0;
 #pragma unroll
         for (int j = 0; j < K / 4; ++j) {
             const uint32_t w4 = 4u * ((uint32_t)j * kThreads + tid);
-            uint32_t x[4];
-            if (vec) {
-                uint4 v = *reinterpret_cast<const uint4*>(kin + base + w4);
-                x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
-            } else {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) x[i] = (w4 + i < nvalid) ? kin[base + w4 + i] : 0u;
-            }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if (vec || w4 + i < nvalid) pc += (uint64_t)1 << (8u * f(x[i]));
+                if (FULL || w4 + i < nvalid) pc += (uint64_t)1 << (8u * f(x[4 * j + i]));
         }
 #pragma unroll
         for (int b = 0; b < B; ++b) {
-            uint32_t c = wave_incl_scan((uint32_t)(pc >> (8 * b)) & 0xFFu);
+            const uint32_t c = wave_incl_scan((uint32_t)(pc >> (8 * b)) & 0xFFu);
             if (lane == 63u) s_h[wid * B + b] = c;
         }
     } else {
-        for (uint32_t i = tid; i < (uint32_t)(kWaves * B); i += kThreads) s_h[i] = 0;
-        __syncthreads();
+        // Per-wave sub-histograms; a wave zeroes and fills only its own.
         uint32_t* my = s_h + wid * B;
+        for (uint32_t i = lane; i < (uint32_t)B; i += 64) my[i] = 0;
+        GMS_WAVE_SYNC();
 #pragma unroll
         for (int j = 0; j < K / 4; ++j) {
             const uint32_t w4 = 4u * ((uint32_t)j * kThreads + tid);
-            uint32_t x[4];
-            if (vec) {
-                uint4 v = *reinterpret_cast<const uint4*>(kin + base + w4);
-                x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
-            } else {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) x[i] = (w4 + i < nvalid) ? kin[base + w4 + i] : 0u;
-            }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if (vec || w4 + i < nvalid) atomicAdd(&my[f(x[i])], 1u);
+                if (FULL || w4 + i < nvalid) atomicAdd(&my[f(x[4 * j + i])], 1u);
         }
     }
     __syncthreads();
     for (uint32_t b = tid; b < (uint32_t)B; b += kThreads) {
         uint32_t s = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) s += s_h[w * B + b];
-        H[(size_t)b * num_tiles + tile] = s;
+        for (int q = 0; q < kWaves; ++q) s += s_h[q * B + b];
+        H[(size_t)b * num_tiles + t] = s;
+    }
+    __syncthreads();
+}
+
+template <int LOG_B, int K, bool VEC, class F>
+__global__ __launch_bounds__(kThreads) void ms_prescan_kernel(const uint32_t* __restrict__ kin,
+                                                              uint32_t n, uint32_t num_tiles,
+                                                              uint32_t* __restrict__ H, F f) {
+    constexpr int B = 1 << LOG_B;
+    constexpr uint32_t TILE = kThreads * K;
+    __shared__ __attribute__((aligned(16))) uint32_t s_h[kWaves * B];
+
+    const uint32_t t = xcd_tile(blockIdx.x, num_tiles);
+    const uint32_t nvalid = tile_valid(n, t, TILE);
+    uint32_t x[K];
+    if (nvalid == TILE) {
+        load_striped4_full<K, VEC>(kin, (size_t)t * TILE, x);
+        prescan_tile<LOG_B, K, true>(x, TILE, t, num_tiles, H, f, s_h);
+    } else {
+        load_striped4_guarded<K>(kin, (size_t)t * TILE, nvalid, x);
+        prescan_tile<LOG_B, K, false>(x, nvalid, t, num_tiles, H, f, s_h);
     }
 }
 
@@ -116,76 +167,82 @@ __global__ __launch_bounds__(kThreads) void ms_prescan_kernel(const uint32_t* __
 // On entry key/val hold the thread's K consecutive keys of the current order
 // (thread t owns positions [t*K, t*K+K)); on exit s_keys/s_vals hold the tile
 // stably reordered by that digit.  Ends with a barrier.
+//
+// s_cnt is D x 256 u32, digit-major: word d*256 + t counts digit d among the
+// keys of thread t.  Counting hits 64 distinct banks per wave; the scan reads
+// D consecutive words per thread, so position = exclusive prefix in memory
+// order = (keys of smaller digits) + (keys of this digit in earlier threads).
 // ---------------------------------------------------------------------------
-template <int SHIFT, int W, int K, bool KV, class F>
+template <int SHIFT, int W, int K, bool KV, bool FULL, class F>
 __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32_t (&val)[K],
                                           uint32_t nvalid, F f, uint32_t* s_keys,
-                                          uint32_t* s_vals, uint32_t* s_cnt, uint32_t* s_wsum) {
-    constexpr int RW = (W >= 1) ? (1 << (W - 1)) : 1;  // packed counter words per thread
-    constexpr uint32_t DMASK = (1u << W) - 1u;
+                                          uint32_t* s_vals, uint32_t* s_cnt, uint32_t* s_ws) {
+    constexpr int D = 1 << W;
+    constexpr uint32_t DMASK = (uint32_t)D - 1u;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
     const uint32_t first = tid * (uint32_t)K;
 
-    // Thread-private column of counters: word r holds digits 2r (low half)
-    // and 2r+1 (high half) of this thread.  Column layout r*256+tid keeps
-    // every ds access of a wave on 64 distinct banks.
 #pragma unroll
-    for (int r = 0; r < RW; ++r) s_cnt[r * kThreads + tid] = 0;
+    for (int d = 0; d < D; ++d) s_cnt[d * kThreads + tid] = 0;
 
     uint32_t lr[K], dg[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         dg[k] = 0;
         lr[k] = 0;
-        if (first + (uint32_t)k < nvalid) {
+        if (FULL || first + (uint32_t)k < nvalid) {
             const uint32_t d = (f(key[k]) >> SHIFT) & DMASK;
-            const uint32_t sh = (d & 1u) << 4;
-            const uint32_t old = atomicAdd(&s_cnt[(d >> 1) * kThreads + tid], 1u << sh);
-            lr[k] = (old >> sh) & 0xFFFFu;
+            lr[k] = atomicAdd(&s_cnt[(d << 8) | tid], 1u);
             dg[k] = d;
         }
     }
+    __syncthreads();
 
-    // Column scan: for every digit, exclusive prefix over threads, plus the
-    // digit's start in the tile.  Both halves of a packed word scan at once
-    // (every partial sum is <= tile size < 2^16).
-    uint32_t c[RW], inc[RW];
+    uint32_t c[D];
+    uint32_t* mine = s_cnt + tid * (uint32_t)D;
+    if constexpr (D % 4 == 0) {
 #pragma unroll
-    for (int r = 0; r < RW; ++r) {
-        c[r] = s_cnt[r * kThreads + tid];
-        inc[r] = wave_incl_scan(c[r]);
-        if (lane == 63u) s_wsum[wid * RW + r] = inc[r];
+        for (int i = 0; i < D / 4; ++i) {
+            const uint4 v = reinterpret_cast<const uint4*>(mine)[i];
+            c[4 * i + 0] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; ++i) c[i] = mine[i];
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) s += c[i];
+    const uint32_t inc = wave_incl_scan(s);
+    if (lane == 63u) s_ws[wid] = inc;
+    __syncthreads();
+    uint32_t run = inc - s;
+#pragma unroll
+    for (int i = 0; i < kWaves; ++i) run += ((uint32_t)i < wid) ? s_ws[i] : 0u;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        const uint32_t v = c[i];
+        c[i] = run;
+        run += v;
+    }
+    if constexpr (D % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < D / 4; ++i)
+            reinterpret_cast<uint4*>(mine)[i] =
+                make_uint4(c[4 * i + 0], c[4 * i + 1], c[4 * i + 2], c[4 * i + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; ++i) mine[i] = c[i];
     }
     __syncthreads();
-    uint32_t run = 0;
-#pragma unroll
-    for (int r = 0; r < RW; ++r) {
-        uint32_t pre = 0, tot = 0;
-#pragma unroll
-        for (int i = 0; i < kWaves; ++i) {
-            const uint32_t v = s_wsum[i * RW + r];
-            pre += ((uint32_t)i < wid) ? v : 0u;
-            tot += v;
-        }
-        const uint32_t ex = pre + inc[r] - c[r];
-        const uint32_t lo_start = run;
-        run += tot & 0xFFFFu;
-        const uint32_t hi_start = run;
-        run += tot >> 16;
-        s_cnt[r * kThreads + tid] = ((lo_start + (ex & 0xFFFFu)) & 0xFFFFu) |
-                                    ((hi_start + (ex >> 16)) << 16);
-    }
 
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        if (first + (uint32_t)k < nvalid) {
-            const uint32_t d = dg[k];
-            const uint32_t v = s_cnt[(d >> 1) * kThreads + tid];
-            const uint32_t pos = ((v >> ((d & 1u) << 4)) & 0xFFFFu) + lr[k];
-            if (pos < nvalid) {  // always true when counts are consistent
-                s_keys[pos] = key[k];
-                if constexpr (KV) s_vals[pos] = val[k];
-            }
+        if (FULL || first + (uint32_t)k < nvalid) {
+            uint32_t pos = s_cnt[(dg[k] << 8) | tid] + lr[k];
+            pos = pos < nvalid ? pos : nvalid - 1u;  // in bounds even if counts were inconsistent
+            s_keys[pos] = key[k];
+            if constexpr (KV) s_vals[pos] = val[k];
         }
     }
     __syncthreads();
@@ -207,86 +264,120 @@ __device__ __forceinline__ void reload_blocked(uint32_t (&key)[K], uint32_t (&va
 }
 
 // ---------------------------------------------------------------------------
-// Postscan: stable in-tile reorder + write-out.
+// Postscan: stable in-tile reorder + write-out, persistent with prefetch.
 // ---------------------------------------------------------------------------
-template <int LOG_B, int K, int LEVEL_BITS, bool KV, class F>
-__global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
-    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, uint32_t num_tiles,
-    const uint32_t* __restrict__ G, F f, int vec_ok) {
+// In-tile ranking of one tile: delta[b] = global start - local start of every
+// bucket run, then the stable LDS passes; leaves the tile in bucket order in
+// s_keys/s_vals.
+template <int LOG_B, int K, int LEVEL_BITS, bool KV, bool FULL, class F>
+__device__ __forceinline__ void postscan_rank(uint32_t (&key)[K], uint32_t (&val)[K], uint32_t g0,
+                                              uint32_t cntb, uint32_t nvalid, F f,
+                                              uint32_t* s_keys, uint32_t* s_vals,
+                                              uint32_t* s_cnt, uint32_t* s_delta, uint32_t* s_bs,
+                                              uint32_t* s_ws) {
     constexpr int B = 1 << LOG_B;
-    constexpr uint32_t TILE = kThreads * K;
     constexpr int NLEV = (LOG_B + LEVEL_BITS - 1) / LEVEL_BITS;
-    constexpr int RW = 1 << (LEVEL_BITS - 1);
-    static_assert(NLEV >= 1 && NLEV <= 3, "at most three ranking passes");
-    static_assert(TILE <= 32768, "packed 16-bit positions");
-    static_assert(K % 4 == 0, "vector loads");
-
-    __shared__ __attribute__((aligned(16))) uint32_t s_keys[TILE];
-    __shared__ __attribute__((aligned(16))) uint32_t s_vals[KV ? TILE : 4];
-    __shared__ uint32_t s_cnt[RW * kThreads];
-    __shared__ uint32_t s_wsum[kWaves * RW];
-    __shared__ uint32_t s_delta[B];
-    __shared__ uint32_t s_bs[kWaves];
-
     const uint32_t tid = threadIdx.x;
-    const uint32_t tile = xcd_tile(blockIdx.x, num_tiles);
-    const size_t base = (size_t)tile * TILE;
-    const uint32_t nvalid = (uint32_t)(((size_t)n - base) < TILE ? ((size_t)n - base) : TILE);
-    const bool vec = vec_ok && nvalid == TILE;
-
-    // Global start of each of this tile's bucket runs, and the tile's count
-    // per bucket as the difference of consecutive scanned entries.
-    uint32_t g0 = 0, cntb = 0;
-    if (tid < (uint32_t)B) {
-        const size_t idx = (size_t)tid * num_tiles + tile;
-        g0 = G[idx];
-        const uint32_t g1 = (idx + 1 < (size_t)B * num_tiles) ? G[idx + 1] : n;
-        cntb = g1 - g0;
-    }
-
-    uint32_t key[K], val[K];
-    load_blocked<K>(kin, base, nvalid, key, vec);
-    if constexpr (KV) load_blocked<K>(vin, base, nvalid, val, vec);
-    else {
-#pragma unroll
-        for (int k = 0; k < K; ++k) val[k] = 0;
-    }
-
     {
         uint32_t tot;
-        const uint32_t ls = block_excl_scan<kWaves>(cntb, s_bs, tot);
+        const uint32_t ls = block_excl_scan<kWaves>(tid < (uint32_t)B ? cntb : 0u, s_bs, tot);
         if (tid < (uint32_t)B) s_delta[tid] = g0 - ls;
     }
-
     constexpr int W0 = LOG_B < LEVEL_BITS ? LOG_B : LEVEL_BITS;
-    rank_pass<0, W0, K, KV>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_wsum);
+    rank_pass<0, W0, K, KV, FULL>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_ws);
     if constexpr (NLEV >= 2) {
         constexpr int S1 = LEVEL_BITS;
         constexpr int W1 = (LOG_B - S1) < LEVEL_BITS ? (LOG_B - S1) : LEVEL_BITS;
         reload_blocked<K, KV>(key, val, s_keys, s_vals);
-        rank_pass<S1, W1, K, KV>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_wsum);
+        rank_pass<S1, W1, K, KV, FULL>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_ws);
     }
     if constexpr (NLEV >= 3) {
         constexpr int S2 = 2 * LEVEL_BITS;
         constexpr int W2 = LOG_B - S2;
         reload_blocked<K, KV>(key, val, s_keys, s_vals);
-        rank_pass<S2, W2, K, KV>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_wsum);
+        rank_pass<S2, W2, K, KV, FULL>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_ws);
     }
+}
 
-    // Write-out in slot order: consecutive threads write consecutive slots,
-    // which are consecutive addresses inside each bucket run.
+// Write-out in slot order: consecutive threads write consecutive slots, which
+// are consecutive addresses inside each bucket run.  The index is clamped
+// into [0, n): a no-op when the scanned matrix is consistent, and every store
+// stays in bounds even if it were not.
+template <int K, bool KV, bool FULL, class F>
+__device__ __forceinline__ void postscan_write(uint32_t nvalid, uint32_t n,
+                                               uint32_t* __restrict__ kout,
+                                               uint32_t* __restrict__ vout, F f,
+                                               const uint32_t* s_keys, const uint32_t* s_vals,
+                                               const uint32_t* s_delta) {
+    const uint32_t tid = threadIdx.x;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         const uint32_t p = (uint32_t)j * kThreads + tid;
-        if (p < nvalid) {
+        if (FULL || p < nvalid) {
             const uint32_t k = s_keys[p];
-            const uint32_t gp = s_delta[f(k)] + p;
-            if (gp < n) {  // always true when the scanned matrix is consistent
-                kout[gp] = k;
-                if constexpr (KV) vout[gp] = s_vals[p];
-            }
+            uint32_t gp = s_delta[f(k)] + p;
+            gp = gp < n ? gp : n - 1u;
+            kout[gp] = k;
+            if constexpr (KV) vout[gp] = s_vals[p];
         }
+    }
+}
+
+template <int LOG_B, int K, int LEVEL_BITS, bool KV, bool VEC, class F>
+__global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, uint32_t num_tiles,
+    const uint32_t* __restrict__ G, F f) {
+    constexpr int B = 1 << LOG_B;
+    constexpr uint32_t TILE = kThreads * K;
+    constexpr int NLEV = (LOG_B + LEVEL_BITS - 1) / LEVEL_BITS;
+    static_assert(NLEV >= 1 && NLEV <= 3, "at most three ranking passes");
+    static_assert(K % 4 == 0, "vector loads");
+
+    __shared__ __attribute__((aligned(16))) uint32_t s_keys[TILE];
+    __shared__ __attribute__((aligned(16))) uint32_t s_vals[KV ? TILE : 4];
+    __shared__ __attribute__((aligned(16))) uint32_t s_cnt[(1 << LEVEL_BITS) * kThreads];
+    __shared__ uint32_t s_delta[B];
+    __shared__ uint32_t s_bs[kWaves];
+    __shared__ uint32_t s_ws[kWaves];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t t = xcd_tile(blockIdx.x, num_tiles);
+    const uint32_t nvalid = tile_valid(n, t, TILE);
+    const size_t base = (size_t)t * TILE;
+
+    // Scanned entry G[b][t] and the next one in scan order (n past the end):
+    // their difference is the tile's count in bucket b.  Loaded by every
+    // thread (threads past B reload bucket B-1 and ignore it).
+    const uint32_t bt = tid < (uint32_t)B ? tid : (uint32_t)B - 1u;
+    const size_t idx = (size_t)bt * num_tiles + t;
+    const size_t mtot = (size_t)B * num_tiles;
+    const uint32_t g0 = G[idx];
+    const uint32_t g1raw = G[idx + 1 < mtot ? idx + 1 : idx];
+
+    uint32_t key[K], val[K];
+    if (nvalid == TILE) {
+        load_blocked_full<K, VEC>(kin, base, key);
+        if constexpr (KV) load_blocked_full<K, VEC>(vin, base, val);
+        else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) val[k] = 0;
+        }
+        const uint32_t cntb = (idx + 1 < mtot ? g1raw : n) - g0;
+        postscan_rank<LOG_B, K, LEVEL_BITS, KV, true>(key, val, g0, cntb, TILE, f, s_keys,
+                                                      s_vals, s_cnt, s_delta, s_bs, s_ws);
+        postscan_write<K, KV, true>(TILE, n, kout, vout, f, s_keys, s_vals, s_delta);
+    } else {
+        load_blocked_guarded<K>(kin, base, nvalid, key);
+        if constexpr (KV) load_blocked_guarded<K>(vin, base, nvalid, val);
+        else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) val[k] = 0;
+        }
+        const uint32_t cntb = (idx + 1 < mtot ? g1raw : n) - g0;
+        postscan_rank<LOG_B, K, LEVEL_BITS, KV, false>(key, val, g0, cntb, nvalid, f, s_keys,
+                                                       s_vals, s_cnt, s_delta, s_bs, s_ws);
+        postscan_write<K, KV, false>(nvalid, n, kout, vout, f, s_keys, s_vals, s_delta);
     }
 }
 

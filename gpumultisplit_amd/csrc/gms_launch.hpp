@@ -6,22 +6,34 @@
 
 namespace gms {
 
+template <int LOG_B, bool KV, bool VEC, class F>
+hipError_t launch_prescan_v(const MsCall& c, const F& f) {
+    constexpr int K = tune_items(LOG_B, KV);
+    hipLaunchKernelGGL((ms_prescan_kernel<LOG_B, K, VEC, F>), dim3(c.num_tiles), dim3(kThreads),
+                       0, c.stream, c.kin, c.n, c.num_tiles, c.H, f);
+    return hipGetLastError();
+}
+
 template <int LOG_B, bool KV, class F>
 hipError_t launch_prescan_t(const MsCall& c, const F& f) {
+    return c.vec_ok ? launch_prescan_v<LOG_B, KV, true>(c, f)
+                    : launch_prescan_v<LOG_B, KV, false>(c, f);
+}
+
+template <int LOG_B, bool KV, bool VEC, class F>
+hipError_t launch_postscan_v(const MsCall& c, const F& f) {
     constexpr int K = tune_items(LOG_B, KV);
-    hipLaunchKernelGGL((ms_prescan_kernel<LOG_B, K, F>), dim3(c.num_tiles), dim3(kThreads), 0,
-                       c.stream, c.kin, c.n, c.num_tiles, c.H, f, c.vec_ok);
+    constexpr int LB = tune_level_bits(LOG_B);
+    hipLaunchKernelGGL((ms_postscan_kernel<LOG_B, K, LB, KV, VEC, F>), dim3(c.num_tiles),
+                       dim3(kThreads), 0, c.stream, c.kin, c.vin, c.kout, c.vout, c.n,
+                       c.num_tiles, c.H, f);
     return hipGetLastError();
 }
 
 template <int LOG_B, bool KV, class F>
 hipError_t launch_postscan_t(const MsCall& c, const F& f) {
-    constexpr int K = tune_items(LOG_B, KV);
-    constexpr int LB = tune_level_bits(LOG_B);
-    hipLaunchKernelGGL((ms_postscan_kernel<LOG_B, K, LB, KV, F>), dim3(c.num_tiles),
-                       dim3(kThreads), 0, c.stream, c.kin, c.vin, c.kout, c.vout, c.n,
-                       c.num_tiles, c.H, f, c.vec_ok);
-    return hipGetLastError();
+    return c.vec_ok ? launch_postscan_v<LOG_B, KV, true>(c, f)
+                    : launch_postscan_v<LOG_B, KV, false>(c, f);
 }
 
 #define GMS_SWITCH_LOGB_KV(FN, c, f)                                   \

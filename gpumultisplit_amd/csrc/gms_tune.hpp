@@ -17,12 +17,14 @@ constexpr int kWaves = kThreads / 64;
 constexpr int tune_items(int log_b, bool kv) {
     (void)log_b;
     (void)kv;
-    return 16;
+    return 24;
 }
 
-// Widest digit ranked in one LDS pass (packed 16-bit counters, 2^(w-1)
-// words per thread).  Up to 5 bits one pass; 6..8 bits take two passes.
-constexpr int tune_level_bits(int log_b) { return log_b <= 5 ? (log_b < 1 ? 1 : log_b) : 4; }
+// Widest digit ranked in one LDS pass (2^w x 256 u32 counters).  Up to 5 bits
+// in one pass; 6 bits as 3+3, 7 as 4+3, 8 as 4+4.
+constexpr int tune_level_bits(int log_b) {
+    return log_b <= 5 ? (log_b < 1 ? 1 : log_b) : (log_b == 6 ? 3 : 4);
+}
 
 constexpr uint32_t tile_keys(int log_b, bool kv) { return (uint32_t)(kThreads * tune_items(log_b, kv)); }
 

@@ -1,0 +1,212 @@
+// Runs the library's C ABI, built from the unmodified HIP sources on the CPU
+// emulation (hip_emu.hpp), under AddressSanitizer + UBSan, over the shapes the
+// GPU tests use, and compares every result with the C oracle.  Exit status 0
+// when every case matches; ASan aborts on any out-of-bounds access.
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "gpumultisplit.h"
+
+extern "C" {
+int ms_oracle_multisplit(uint32_t kind, uint32_t B, uint32_t param, const uint32_t* kin,
+                         const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint64_t n,
+                         uint32_t* offsets);
+int ms_oracle_sort(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
+                   uint64_t n, uint32_t digit_bits);
+void ms_oracle_histogram(uint32_t kind, uint32_t B, uint32_t param, const uint32_t* keys,
+                         uint64_t n, uint32_t* hist);
+uint64_t ms_oracle_compaction(uint32_t kind, uint32_t param, const uint32_t* kin,
+                              const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint64_t n);
+void ms_oracle_tile_histograms(uint32_t kind, uint32_t B, uint32_t param, const uint32_t* keys,
+                               uint64_t n, uint32_t tile, uint32_t* H);
+}
+
+static uint64_t rng_state = 0x1234567;
+static uint32_t rnd() {
+    uint64_t z = (rng_state += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (uint32_t)(z ^ (z >> 31));
+}
+
+static int failures = 0;
+#define EXPECT(cond, ...)                   \
+    do {                                    \
+        if (!(cond)) {                      \
+            printf("FAIL: " __VA_ARGS__);   \
+            printf("\n");                   \
+            ++failures;                     \
+        }                                   \
+    } while (0)
+
+// Exact-size heap buffers so that ASan sees any access past n.
+struct Buf {
+    uint32_t* p;
+    size_t n;
+    explicit Buf(size_t n_, uint32_t fill = 0x5A5A5A5Au) : n(n_) {
+        p = (uint32_t*)malloc((n ? n : 1) * sizeof(uint32_t));
+        for (size_t i = 0; i < n; ++i) p[i] = fill;
+    }
+    ~Buf() { free(p); }
+    Buf(const Buf&) = delete;
+};
+
+static void multisplit_case(uint32_t B, uint32_t n, gms_bucket_desc d, uint32_t oracle_param,
+                            bool kv, uint32_t key_mod, int misalign) {
+    Buf kin(n + 4), vin(n + 4), kout(n + 4), vout(n + 4);
+    for (uint32_t i = 0; i < n + 4; ++i) {
+        kin.p[i] = key_mod ? rnd() % key_mod : rnd();
+        vin.p[i] = i;
+    }
+    const uint32_t* ki = kin.p + misalign;
+    const uint32_t* vi = vin.p + (misalign ? 3 : 0);
+    uint32_t* ko = kout.p + (misalign ? 2 : 0);
+    uint32_t* vo = vout.p + (misalign ? 1 : 0);
+    gms_multisplit_ctx* ctx = nullptr;
+    EXPECT(gms_multisplit_create(B, &ctx) == GMS_SUCCESS, "create B=%u", B);
+    EXPECT(gms_multisplit_allocate(ctx, n, kv) == GMS_SUCCESS, "allocate");
+    std::vector<uint32_t> offs(B, 77);
+    gms_status st = kv ? gms_multisplit_key_value(ctx, ki, vi, ko, vo, n, d, offs.data(), nullptr)
+                       : gms_multisplit_key_only(ctx, ki, ko, n, d, offs.data(), nullptr);
+    EXPECT(st == GMS_SUCCESS, "multisplit status %d", (int)st);
+    std::vector<uint32_t> wk(n + 1), wv(n + 1), wo(B);
+    ms_oracle_multisplit(d.kind, B, oracle_param, ki, kv ? vi : nullptr, wk.data(),
+                         kv ? wv.data() : nullptr, n, wo.data());
+    bool ok = memcmp(ko, wk.data(), n * 4) == 0 && memcmp(offs.data(), wo.data(), B * 4) == 0 &&
+              (!kv || memcmp(vo, wv.data(), n * 4) == 0);
+    EXPECT(ok, "multisplit kind=%u B=%u n=%u kv=%d misalign=%d", d.kind, B, n, (int)kv, misalign);
+    gms_multisplit_destroy(ctx);
+}
+
+int main(int argc, char** argv) {
+    const bool quick = argc > 1 && strcmp(argv[1], "quick") == 0;
+    const uint32_t tile_kv = gms_subproblem_size(256, 1);
+    const uint32_t sizes[] = {0, 1, 2, 63, 64, 65, 1000, tile_kv - 1, tile_kv, tile_kv + 1,
+                              3 * tile_kv + 17};
+    const uint32_t bs[] = {1, 2, 3, 4, 5, 8, 16, 17, 32, 33, 64, 100, 128, 129, 255, 256};
+    for (uint32_t B : bs) {
+        for (uint32_t n : sizes) {
+            if (quick && n > tile_kv + 1) continue;
+            for (int kv = 0; kv < 2; ++kv)
+                multisplit_case(B, n, gms_bucket_desc{GMS_BUCKET_RANGE, 0, 0}, 0, kv, 0, 0);
+        }
+        printf("range B=%u done\n", B);
+        fflush(stdout);
+    }
+    // other identifiers
+    multisplit_case(32, 20000, gms_bucket_desc{GMS_BUCKET_DELTA, 625, 0}, 625, true, 20000, 0);
+    for (uint32_t delta : {1u, 3u, 1000u, 0x01000001u, 0xFFFFFFFFu})
+        multisplit_case(7, 9000, gms_bucket_desc{GMS_BUCKET_DELTA, delta, 0}, delta, true, 0, 0);
+    multisplit_case(16, 9000, gms_bucket_desc{GMS_BUCKET_IDENTITY, 0, 0}, 0, true, 16, 0);
+    multisplit_case(16, 9000, gms_bucket_desc{GMS_BUCKET_IDENTITY, 0, 0}, 0, true, 32, 0);
+    multisplit_case(256, 9000, gms_bucket_desc{GMS_BUCKET_DIGIT, 24, 8}, 24, true, 0, 0);
+    multisplit_case(128, 9000, gms_bucket_desc{GMS_BUCKET_DIGIT, 25, 7}, 25, true, 0, 0);
+    multisplit_case(8, 9000, gms_bucket_desc{GMS_BUCKET_LOWBITS, 3, 0}, 3, false, 0, 0);
+    multisplit_case(128, 2 * tile_kv + 5, gms_bucket_desc{GMS_BUCKET_RANGE, 0, 0}, 0, true, 0, 1);
+    multisplit_case(256, 2 * tile_kv, gms_bucket_desc{GMS_BUCKET_RANGE, 0, 0}, 0, false, 0, 1);
+    printf("identifiers done\n");
+    fflush(stdout);
+
+    // prescan matrix and scan stage
+    {
+        const uint32_t n = 3 * tile_kv + 5, B = 256;
+        Buf keys(n);
+        for (uint32_t i = 0; i < n; ++i) keys.p[i] = rnd();
+        gms_multisplit_ctx* ctx;
+        gms_multisplit_create(B, &ctx);
+        gms_multisplit_allocate(ctx, n, 1);
+        const uint32_t nt = (n + tile_kv - 1) / tile_kv;
+        Buf mat(B * nt);
+        std::vector<uint32_t> want(B * nt);
+        EXPECT(gms_multisplit_prescan_matrix(ctx, keys.p, n, gms_bucket_desc{0, 0, 0}, 1, mat.p,
+                                             nullptr) == GMS_SUCCESS,
+               "prescan matrix status");
+        ms_oracle_tile_histograms(0, B, 0, keys.p, n, tile_kv, want.data());
+        EXPECT(memcmp(mat.p, want.data(), B * nt * 4) == 0, "prescan matrix");
+        for (uint32_t m : {1u, 5u, 4095u, 4096u, 4097u, 20000u}) {
+            if (m > B * nt) continue;
+            Buf in(m), out(m);
+            uint64_t run = 0;
+            std::vector<uint32_t> ex(m);
+            for (uint32_t i = 0; i < m; ++i) {
+                in.p[i] = rnd() % 1000;
+                ex[i] = (uint32_t)run;
+                run += in.p[i];
+            }
+            EXPECT(gms_exclusive_scan_u32(ctx, in.p, out.p, m, nullptr) == GMS_SUCCESS, "scan st");
+            EXPECT(memcmp(out.p, ex.data(), m * 4) == 0, "scan m=%u", m);
+            EXPECT(gms_exclusive_scan_u32(ctx, in.p, in.p, m, nullptr) == GMS_SUCCESS, "scan st");
+            EXPECT(memcmp(in.p, ex.data(), m * 4) == 0, "scan in place m=%u", m);
+        }
+        gms_multisplit_destroy(ctx);
+        printf("stages done\n");
+        fflush(stdout);
+    }
+
+    // sort
+    for (uint32_t bits : {4u, 5u, 6u, 7u, 8u}) {
+        for (uint32_t n : {0u, 1u, 1000u, tile_kv + 3}) {
+            for (int kv = 0; kv < 2; ++kv) {
+                Buf kin(n), vin(n), kout(n), vout(n);
+                for (uint32_t i = 0; i < n; ++i) {
+                    kin.p[i] = (i % 7 == 0) ? 12345u : rnd();
+                    vin.p[i] = i;
+                }
+                gms_sort_ctx* ctx;
+                gms_sort_create(&ctx);
+                EXPECT(gms_sort_allocate(ctx, n, bits, kv) == GMS_SUCCESS, "sort alloc");
+                EXPECT(gms_sort(ctx, kin.p, kout.p, kv ? vin.p : nullptr, kv ? vout.p : nullptr, n,
+                                bits, nullptr) == GMS_SUCCESS,
+                       "sort status");
+                std::vector<uint32_t> wk(n + 1), wv(n + 1);
+                ms_oracle_sort(kin.p, kv ? vin.p : nullptr, wk.data(), kv ? wv.data() : nullptr,
+                               n, bits);
+                EXPECT(memcmp(kout.p, wk.data(), n * 4) == 0 &&
+                           (!kv || memcmp(vout.p, wv.data(), n * 4) == 0),
+                       "sort bits=%u n=%u kv=%d", bits, n, kv);
+                gms_sort_destroy(ctx);
+            }
+        }
+    }
+    printf("sort done\n");
+    fflush(stdout);
+
+    // compaction and histogram
+    for (uint32_t n : {0u, 1u, 5000u, tile_kv + 9}) {
+        Buf kin(n), vin(n), kout(n), vout(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            kin.p[i] = rnd();
+            vin.p[i] = i;
+        }
+        gms_compaction_ctx* c;
+        gms_compaction_create(&c);
+        gms_compaction_allocate(c, n, 1);
+        uint32_t cnt = 99;
+        EXPECT(gms_compaction(c, kin.p, vin.p, kout.p, vout.p, n,
+                              gms_bucket_desc{GMS_BUCKET_LOWBITS, 1, 0}, &cnt, nullptr,
+                              nullptr) == GMS_SUCCESS,
+               "compaction status");
+        std::vector<uint32_t> wk(n + 1), wv(n + 1);
+        uint64_t wc = ms_oracle_compaction(GMS_BUCKET_LOWBITS, 1, kin.p, vin.p, wk.data(),
+                                           wv.data(), n);
+        EXPECT(cnt == wc && memcmp(kout.p, wk.data(), n * 4) == 0 &&
+                   memcmp(vout.p, wv.data(), n * 4) == 0,
+               "compaction n=%u", n);
+        gms_compaction_destroy(c);
+
+        Buf hist(256, 0);
+        std::vector<uint32_t> wh(256, 0);
+        EXPECT(gms_histogram_u32(kin.p, n, hist.p, 256, gms_bucket_desc{GMS_BUCKET_DIGIT, 0, 8},
+                                 nullptr) == GMS_SUCCESS,
+               "histogram status");
+        ms_oracle_histogram(GMS_BUCKET_DIGIT, 256, 0, kin.p, n, wh.data());
+        EXPECT(memcmp(hist.p, wh.data(), 256 * 4) == 0, "histogram n=%u", n);
+    }
+    printf("compaction/histogram done\n");
+    printf(failures ? "EMU CHECK FAILED (%d)\n" : "EMU CHECK OK\n", failures);
+    return failures ? 1 : 0;
+}

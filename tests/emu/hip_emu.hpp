@@ -1,0 +1,134 @@
+// Host emulation of the HIP features the library's kernels and host code use,
+// so that the unmodified kernel sources build with g++ and run on the CPU
+// (test infrastructure: bounds checking with AddressSanitizer and CPU parity
+// tests of the real kernel code at small sizes).
+//
+// Execution model: a kernel launch runs its workgroups one after another;
+// every workgroup is blockDim.x host threads that meet at __syncthreads()
+// (a std::barrier).  __shared__ arrays become function-static storage, which
+// is what LDS is to a sequence of workgroups.  DPP lane moves and readlane
+// are wave collectives: the 64 threads of a wave publish their values and
+// meet at a wave barrier.  All kernel call sites of these collectives are in
+// uniform control flow, as on the GPU.
+#pragma once
+
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <barrier>
+#include <chrono>
+#include <cstdio>
+#include <functional>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(x)
+#define __shared__ static
+
+struct gms_emu_dim3 {
+    uint32_t x, y, z;
+};
+typedef struct dim3 {
+    uint32_t x, y, z;
+    dim3(uint32_t a = 1, uint32_t b = 1, uint32_t c = 1) : x(a), y(b), z(c) {}
+} dim3;
+
+struct alignas(16) uint4 {
+    uint32_t x, y, z, w;
+};
+inline uint4 make_uint4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) { return uint4{a, b, c, d}; }
+
+namespace gms_emu {
+extern thread_local gms_emu_dim3 thread_idx;
+extern gms_emu_dim3 block_idx, block_dim, grid_dim;
+void syncthreads();
+uint32_t update_dpp(uint32_t old, uint32_t src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl);
+int readlane(int x, int lane);
+void wave_sync();
+void launch(dim3 grid, dim3 block, std::function<void()> body);
+}  // namespace gms_emu
+
+#define threadIdx (gms_emu::thread_idx)
+#define blockIdx (gms_emu::block_idx)
+#define blockDim (gms_emu::block_dim)
+#define gridDim (gms_emu::grid_dim)
+#define __syncthreads() gms_emu::syncthreads()
+
+inline uint32_t atomicAdd(uint32_t* p, uint32_t v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline uint32_t __umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
+inline float __fmul_rn(float a, float b) {
+    volatile float r = a * b;
+    return r;
+}
+
+// ---- runtime API -----------------------------------------------------------
+typedef enum hipError_t {
+    hipSuccess = 0,
+    hipErrorInvalidValue = 1,
+    hipErrorOutOfMemory = 2,
+    hipErrorNoDevice = 100
+} hipError_t;
+typedef enum hipMemcpyKind {
+    hipMemcpyHostToHost = 0,
+    hipMemcpyHostToDevice = 1,
+    hipMemcpyDeviceToHost = 2,
+    hipMemcpyDeviceToDevice = 3,
+    hipMemcpyDefault = 4
+} hipMemcpyKind;
+typedef void* hipStream_t;
+struct gms_emu_event {
+    std::chrono::steady_clock::time_point t;
+};
+typedef gms_emu_event* hipEvent_t;
+
+inline hipError_t hipGetDevice(int* d) {
+    *d = 0;
+    return hipSuccess;
+}
+inline hipError_t hipMalloc(void** p, size_t bytes) {
+    *p = aligned_alloc(256, ((bytes + 255) / 256) * 256 + 256);
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+template <class T>
+inline hipError_t hipMalloc(T** p, size_t bytes) {
+    return hipMalloc((void**)p, bytes);
+}
+inline hipError_t hipFree(void* p) {
+    free(p);
+    return hipSuccess;
+}
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t b, hipMemcpyKind, hipStream_t) {
+    memmove(d, s, b);
+    return hipSuccess;
+}
+inline hipError_t hipMemcpy(void* d, const void* s, size_t b, hipMemcpyKind) {
+    memmove(d, s, b);
+    return hipSuccess;
+}
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t* e) {
+    *e = new gms_emu_event();
+    return hipSuccess;
+}
+inline hipError_t hipEventDestroy(hipEvent_t e) {
+    delete e;
+    return hipSuccess;
+}
+inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t = nullptr) {
+    e->t = std::chrono::steady_clock::now();
+    return hipSuccess;
+}
+inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
+    *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
+    return hipSuccess;
+}
+
+#define hipLaunchKernelGGL(kern, grid, block, shmem, stream, ...) \
+    gms_emu::launch(dim3(grid), dim3(block), [=]() { kern(__VA_ARGS__); })

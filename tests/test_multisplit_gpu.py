@@ -222,12 +222,13 @@ def test_prescan_matrix_stage(dev):
 def test_exclusive_scan_stage(dev, m):
     import torch
     ctx = gms.multisplit_context(256)
-    gms.multisplit_allocate_key_value(max(m, 4096) * 2, ctx)   # scratch for m partials
+    gms.multisplit_allocate_key_value(m * 32 + 4096, ctx)   # matrix (and scan scratch) >= m
     x = (W.uniform_keys(m, seed=m) % 1000).astype(np.uint32)
     d_in = to_dev(x, dev)
     d_out = torch.zeros(m, dtype=torch.int32, device=dev)
     gms.exclusive_scan(d_in, d_out, m, ctx)
-    want = (np.concatenate([[0], np.cumsum(x.astype(np.uint64))[:-1]]) & 0xFFFFFFFF).astype(np.uint32)
+    csum = np.cumsum(x.astype(np.uint64))
+    want = (np.concatenate([np.zeros(1, np.uint64), csum[:-1]]) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
     np.testing.assert_array_equal(from_dev(d_out), want)
     gms.exclusive_scan(d_in, d_in, m, ctx)                    # in place
     np.testing.assert_array_equal(from_dev(d_in), want)
@@ -286,6 +287,9 @@ def test_full_size_256_kv_properties(dev, n, dist):
     ko = kout.to(torch.int64) & 0xFFFFFFFF
     bid_out = (ko * B) >> 32
     vo = vout.to(torch.int64)
+    # Range check first: the values index device tensors below, and an
+    # out-of-range index would fault instead of failing the test.
+    assert bool(((vo >= 0) & (vo < n)).all()), "output values outside [0, n)"
     # values are a permutation of 0..n-1
     seen = torch.zeros(n, dtype=torch.int32, device=dev)
     seen.index_add_(0, vo, torch.ones(n, dtype=torch.int32, device=dev))
