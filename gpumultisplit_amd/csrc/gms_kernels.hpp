@@ -25,9 +25,14 @@
 
 namespace gms {
 
+// Keys of tile t.  32-bit arithmetic on purpose (t * tile < n < 2^32 for
+// every launched tile): a uniform 64-bit unsigned compare has no scalar
+// instruction on gfx950, and in that form the compiler has been seen to pick
+// the result with a condition flag clobbered by an address add, which made
+// the ragged last tile count as full.
 __device__ __forceinline__ uint32_t tile_valid(uint32_t n, uint32_t t, uint32_t tile) {
-    const size_t base = (size_t)t * tile;
-    return (uint32_t)(((size_t)n - base) < tile ? ((size_t)n - base) : tile);
+    const uint32_t rem = n - t * tile;
+    return rem < tile ? rem : tile;
 }
 
 // Tile loads.  Full tiles use unconditional loads (16-byte vectors when VEC),
