@@ -1,0 +1,183 @@
+"""GPU parity of the multisplit radix sort, histogram and compaction (C ABI).
+
+Sort: the reference checks its multisplit sort against CUB's radix sort
+(src/main/main_sort.cu:162-229, main_sort_pairs.cu); here the check is the
+CPU oracle (LSD over the same digit schedule) and numpy's stable sort, with
+values = original index so that instability shows.  Histogram: the
+reference compares against CUB's HistogramEven (src/main/main_histogram.cu:
+185-186); here numpy bincount / the oracle's float formula.  Compaction: the
+reference compares with its CPU multisplit (src/main/main_compaction.cu:
+102-115, 167-181); here the oracle's stable partition and its count.
+"""
+import numpy as np
+import pytest
+
+import gpumultisplit_amd as gms
+import oracle
+from gpumultisplit_amd import workloads as W
+from gpu_util import banner, filled_u32, from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def dev_in(a, dev):
+    """Input buffer; an empty input still gets a (never read) one-element buffer."""
+    return to_dev(a, dev) if a.size else filled_u32(0, dev)
+
+SORT_FUNCS = {
+    4: (gms.multisplit_sort_4bit_allocate, gms.multisplit_sort_4bit_allocate_pairs,
+        gms.multisplit_sort_4bit, gms.multisplit_sort_4bit_pairs),
+    5: (gms.multisplit_sort_5bit_allocate, gms.multisplit_sort_5bit_allocate_pairs,
+        gms.multisplit_sort_5bit, gms.multisplit_sort_5bit_pairs),
+    6: (gms.multisplit_sort_6bit_allocate, gms.multisplit_sort_6bit_allocate_pairs,
+        gms.multisplit_sort_6bit, gms.multisplit_sort_6bit_pairs),
+    7: (gms.multisplit_sort_7bit_allocate, gms.multisplit_sort_7bit_allocate_pairs,
+        gms.multisplit_sort_7bit, gms.multisplit_sort_7bit_pairs),
+    8: (gms.multisplit_sort_8bit_allocate, gms.multisplit_sort_8bit_allocate_pairs,
+        gms.multisplit_sort_8bit, gms.multisplit_sort_8bit_pairs),
+}
+
+
+def gpu_sort(dev, keys, bits, kv):
+    n = keys.size
+    alloc, alloc_pairs, sort, sort_pairs = SORT_FUNCS[bits]
+    ctx = gms.ms_sort_context()
+    (alloc_pairs if kv else alloc)(n, ctx)
+    kin = dev_in(keys, dev)
+    kout = filled_u32(n, dev)
+    if kv:
+        vin = dev_in(np.arange(n, dtype=np.uint32), dev)
+        vout = filled_u32(n, dev)
+        sort_pairs(kin, kout, vin, vout, n, ctx)
+        res = from_dev(kout)[:n], from_dev(vout)[:n]
+    else:
+        sort(kin, kout, n, ctx)
+        res = from_dev(kout)[:n], None
+    # the input is never written
+    np.testing.assert_array_equal(from_dev(kin)[:n], keys)
+    gms.multisplit_sort_release_memory(ctx)
+    return res
+
+
+@pytest.mark.parametrize("bits", [4, 5, 6, 7, 8])
+@pytest.mark.parametrize("kv", [False, True])
+def test_sort_all_digit_widths(dev, bits, kv):
+    for n in (0, 1, 7, 6143, 6145, 100_003):
+        banner(f"sort bits={bits} kv={kv} n={n}")
+        keys = W.uniform_keys(n, seed=bits * 1000 + n)
+        if n > 10:
+            keys[::9] = keys[3]               # duplicates expose instability
+        got = gpu_sort(dev, keys, bits, kv)
+        wk, wv = oracle.sort(keys, bits, np.arange(n, dtype=np.uint32) if kv else None)
+        np.testing.assert_array_equal(got[0], wk)
+        if kv:
+            np.testing.assert_array_equal(got[1], wv)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("bits", [7, 8])
+@pytest.mark.parametrize("kv", [False, True])
+def test_sort_2e25_like_reference_config(dev, bits, kv):
+    # 2^25 uint32 keys uniform, key-only and key-value (value = original index)
+    n = 1 << 25
+    keys = W.uniform_keys(n, seed=bits + 25)
+    got = gpu_sort(dev, keys, bits, kv)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(got[0], keys[order])
+    if kv:
+        np.testing.assert_array_equal(got[1], order.astype(np.uint32))
+
+
+def test_sort_reference_driver_keys(dev):
+    # main_sort.cu feeds rand() keys (31 bits) and pairs with value = key
+    n = 1 << 20
+    keys = (W.uniform_keys(n, seed=31) >> np.uint32(1)).astype(np.uint32)
+    got = gpu_sort(dev, keys, 7, True)
+    np.testing.assert_array_equal(got[0], np.sort(keys))
+
+
+@pytest.mark.parametrize("num_bins", [2, 32, 64, 100, 256])
+def test_histogram_u32_bins(dev, num_bins):
+    import torch
+    for n in (0, 1, 4095, 123_457):
+        keys = W.uniform_keys(n, seed=num_bins + n)
+        h = torch.zeros(num_bins, dtype=torch.int32, device=dev)
+        gms.histogram(dev_in(keys, dev), n, h, num_bins, gms.range_bucket(num_bins))
+        np.testing.assert_array_equal(from_dev(h), oracle.histogram(keys, num_bins, oracle.RANGE))
+
+
+def test_histogram_256_low_byte_accumulates(dev):
+    import torch
+    # 256-bin histogram of the low 8 bits, added twice into the same array
+    n = (1 << 22) + 5
+    keys = W.uniform_keys(n, seed=256)
+    h = torch.zeros(256, dtype=torch.int32, device=dev)
+    d = to_dev(keys, dev)
+    gms.histogram(d, n, h, 256, gms.ms_bucket_sort_8bit(0))
+    gms.histogram(d, n, h, 256, gms.ms_bucket_sort_8bit(0))
+    want = 2 * np.bincount(keys & 0xFF, minlength=256)
+    np.testing.assert_array_equal(from_dev(h).astype(np.int64), want)
+
+
+def test_histogram_even_float_like_reference_driver(dev):
+    import torch
+    # main_histogram.cu:40-120: float samples in [0, 256), 32 bins, inv_delta = 32/256
+    n = 1 << 20
+    g = W.rng(7)
+    x = (g.random(n, dtype=np.float32) * np.float32(256.0)).astype(np.float32)
+    x[:4] = np.array([0.0, 255.99998, 256.0, -0.5], dtype=np.float32)
+    lower, upper, bins = 0.0, 256.0, 32
+    inv = float(np.float32(1.0) / np.float32((upper - lower) / bins))
+    h = torch.zeros(bins, dtype=torch.int32, device=dev)
+    gms.multisplit2_histogram_even(torch.from_numpy(x).to(dev), n, h, lower, upper, inv, bins)
+    np.testing.assert_array_equal(from_dev(h).view(np.int32),
+                                  oracle.histogram_even_f32(x, bins, lower, inv))
+
+
+@pytest.mark.parametrize("selectivity", [0.5, 0.05])
+@pytest.mark.parametrize("kv", [False, True])
+def test_compaction_selectivity(dev, selectivity, kv):
+    # predicate key < p * 2^32: bucket 0 = kept; built as a 2-bucket delta identifier
+    n = (1 << 20) + 3
+    keys = W.uniform_keys(n, seed=int(selectivity * 100) + kv)
+    thr = int(selectivity * (1 << 32))
+    f = gms.delta_func(thr)                       # key / thr: 0 below thr, clamped 1 above
+    ctx = gms.compaction_context()
+    (gms.compaction_allocate_key_value if kv else gms.compaction_allocate_key_only)(n, ctx)
+    kin = to_dev(keys, dev)
+    kout = filled_u32(n, dev)
+    vals = np.arange(n, dtype=np.uint32)
+    if kv:
+        vout = filled_u32(n, dev)
+        cnt = gms.compaction_key_value(kin, to_dev(vals, dev), kout, vout, n, ctx, f)
+    else:
+        cnt = gms.compaction_key_only(kin, kout, n, ctx, f)
+    gms.compaction_release_memory(ctx)
+    keep = keys < np.uint32(thr) if thr < (1 << 32) else np.ones(n, bool)
+    order = np.concatenate([np.nonzero(keep)[0], np.nonzero(~keep)[0]])
+    assert cnt == int(keep.sum())
+    np.testing.assert_array_equal(from_dev(kout)[:n], keys[order])
+    if kv:
+        np.testing.assert_array_equal(from_dev(vout)[:n], vals[order])
+
+
+def test_compaction_like_reference_driver_and_device_count(dev):
+    import torch
+    # main_compaction.cu: keys = permutation of 0..n-1, bucket = key & 1
+    for n in (0, 1, 6145, 1 << 18):
+        keys = W.permutation_keys(n, seed=n + 1)
+        ctx = gms.compaction_context()
+        gms.compaction_allocate_key_only(n, ctx)
+        kin = dev_in(keys, dev)
+        kout = filled_u32(n, dev)
+        cnt = gms.compaction_key_only(kin, kout, n, ctx, gms.test_bucket(1))
+        c2, wk, _ = oracle.compaction(keys, oracle.LOWBITS, 1)
+        assert cnt == c2
+        np.testing.assert_array_equal(from_dev(kout)[:n], wk)
+        d_cnt = torch.full((1,), -1, dtype=torch.int32, device=dev)
+        kout2 = filled_u32(n, dev)
+        gms.compaction_device_count(kin, kout2, n, ctx, gms.test_bucket(1), d_cnt)
+        torch.cuda.synchronize()
+        assert int(d_cnt.item()) == c2
+        np.testing.assert_array_equal(from_dev(kout2)[:n], wk)
+        gms.compaction_release_memory(ctx)
