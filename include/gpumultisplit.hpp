@@ -1,0 +1,353 @@
+// gpumultisplit.hpp -- the reference's C++ API rebuilt on the C ABI of
+// libgpumultisplit.so (gpumultisplit.h).
+//
+// Same names, argument order and meaning as the reference's header-only API
+// (src/api/multisplit.cuh, multisplit_sort.cuh, multisplit_sort_pairs.cuh,
+// src/kernels/compaction/multisplit2_compaction.cuh), so a caller switches by
+// changing its include.  Differences, all deliberate:
+//  * Errors throw gms::error (the reference returns void and checks nothing).
+//  * The call-time n may be anything up to the allocate-time n (the reference
+//    reuses the grid frozen at allocate time, so a different n breaks).
+//  * Every call takes an optional trailing stream (hipStream_t converts to
+//    void*); the default is the null stream, as in the reference.
+//  * Contexts free their device memory in the destructor as well (the
+//    reference's destructors are no-ops); *_release_memory still works.
+//  * A bucket functor crosses the C ABI as a gms_bucket_desc.  The
+//    reference's functors are provided (ms_bucket_sort_Nbit at global scope
+//    as in multisplit_sort.cuh; delta_func, identity_bucket, test_bucket --
+//    which the reference defines in its drivers -- and range_bucket in
+//    namespace gms).  A caller's own functor type is accepted after
+//    specialising gms::bucket_traits<T> with
+//      static gms_bucket_desc desc(const T&, uint32_t num_buckets);
+//    mapping it to one of the built-in kinds.
+//  * KeyT and ValueT must be 32-bit (the reference stages keys and values as
+//    uint32_t in shared memory, e.g. bms_postscan_pairs.cuh:1000).
+#ifndef GPUMULTISPLIT_HPP_
+#define GPUMULTISPLIT_HPP_
+
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+#include "gpumultisplit.h"
+
+namespace gms {
+
+class error : public std::runtime_error {
+  public:
+    error(gms_status s, const char* where)
+        : std::runtime_error(std::string(where) + ": " + gms_status_string(s)), status(s) {}
+    gms_status status;
+};
+
+inline void check(gms_status s, const char* where) {
+    if (s != GMS_SUCCESS) throw error(s, where);
+}
+
+// ---------------------------------------------------------------- functors
+// key / delta_ (src/main/main_multisplit.cu:41-47).
+struct delta_func {
+    delta_func(uint32_t delta) : delta_(delta) {}
+    uint32_t delta_;
+    uint32_t operator()(uint32_t a) const { return a / delta_; }
+};
+
+// key itself (src/main/main_multisplit.cu:34-39).
+template <typename key_type>
+struct identity_bucket {
+    uint32_t operator()(key_type a) const { return uint32_t(a); }
+};
+
+// key & ((1 << flag) - 1) (src/main/main_compaction.cu:27-32).
+template <uint32_t flag, typename key_type>
+struct test_bucket {
+    uint32_t operator()(key_type a) const { return a & ((1u << flag) - 1); }
+};
+
+// floor(key * B / 2^32): B equal key ranges.
+struct range_bucket {
+    explicit range_bucket(uint32_t num_buckets) : num_buckets_(num_buckets) {}
+    uint32_t num_buckets_;
+    uint32_t operator()(uint32_t a) const {
+        return (uint32_t)(((uint64_t)a * num_buckets_) >> 32);
+    }
+};
+
+template <class T, class Enable = void>
+struct bucket_traits {
+    static_assert(sizeof(T) == 0,
+                  "no gms_bucket_desc for this functor type: specialise gms::bucket_traits");
+};
+
+template <>
+struct bucket_traits<delta_func> {
+    static gms_bucket_desc desc(const delta_func& f, uint32_t) {
+        return gms_bucket_desc{GMS_BUCKET_DELTA, f.delta_, 0};
+    }
+};
+template <typename K>
+struct bucket_traits<identity_bucket<K>> {
+    static gms_bucket_desc desc(const identity_bucket<K>&, uint32_t) {
+        return gms_bucket_desc{GMS_BUCKET_IDENTITY, 0, 0};
+    }
+};
+template <uint32_t flag, typename K>
+struct bucket_traits<test_bucket<flag, K>> {
+    static gms_bucket_desc desc(const test_bucket<flag, K>&, uint32_t) {
+        return gms_bucket_desc{GMS_BUCKET_LOWBITS, flag, 0};
+    }
+};
+template <>
+struct bucket_traits<range_bucket> {
+    static gms_bucket_desc desc(const range_bucket& f, uint32_t num_buckets) {
+        if (f.num_buckets_ != num_buckets)
+            throw error(GMS_ERR_INVALID_ARG, "range_bucket: bucket count differs from the context's");
+        return gms_bucket_desc{GMS_BUCKET_RANGE, 0, 0};
+    }
+};
+
+template <class F>
+inline gms_bucket_desc desc_of(const F& f, uint32_t num_buckets) {
+    return bucket_traits<typename std::decay<F>::type>::desc(f, num_buckets);
+}
+
+template <class T>
+inline void require_32bit(const char* what) {
+    static_assert(sizeof(T) == 4, "keys and values must be 32-bit");
+    (void)what;
+}
+
+}  // namespace gms
+
+// (key >> x) & (2^N - 1) (src/api/multisplit_sort.cuh:63-98).
+#define GMS_DIGIT_FUNCTOR(N, MASK)                                                      \
+    template <uint32_t x, typename key_type>                                            \
+    struct ms_bucket_sort_##N##bit {                                                    \
+        uint32_t operator()(key_type a) const { return uint32_t((a >> x) & MASK); }     \
+    };                                                                                  \
+    namespace gms {                                                                     \
+    template <uint32_t x, typename K>                                                   \
+    struct bucket_traits<::ms_bucket_sort_##N##bit<x, K>> {                             \
+        static gms_bucket_desc desc(const ::ms_bucket_sort_##N##bit<x, K>&, uint32_t) { \
+            return gms_bucket_desc{GMS_BUCKET_DIGIT, x, N};                             \
+        }                                                                               \
+    };                                                                                  \
+    }
+GMS_DIGIT_FUNCTOR(2, 0x03)
+GMS_DIGIT_FUNCTOR(4, 0x0F)
+GMS_DIGIT_FUNCTOR(5, 0x1F)
+GMS_DIGIT_FUNCTOR(6, 0x3F)
+GMS_DIGIT_FUNCTOR(7, 0x7F)
+GMS_DIGIT_FUNCTOR(8, 0xFF)
+#undef GMS_DIGIT_FUNCTOR
+
+// ---------------------------------------------------------------- multisplit
+// src/api/multisplit.cuh:29-47.
+class multisplit_context {
+  public:
+    uint32_t size_sub_prob = 0;  // keys per tile of the last allocation
+    uint32_t num_sub_prob = 0;   // tiles at the allocated capacity
+    uint32_t num_buckets;
+    gms_multisplit_ctx* handle = nullptr;
+
+    explicit multisplit_context(uint32_t n_buckets) : num_buckets(n_buckets) {
+        gms::check(gms_multisplit_create(n_buckets, &handle), "multisplit_context");
+    }
+    ~multisplit_context() { gms_multisplit_destroy(handle); }
+    multisplit_context(const multisplit_context&) = delete;
+    multisplit_context& operator=(const multisplit_context&) = delete;
+};
+
+namespace gms {
+inline void ms_allocate(uint32_t n, multisplit_context& ctx, int kv, const char* what) {
+    check(gms_multisplit_allocate(ctx.handle, n, kv), what);
+    ctx.size_sub_prob = gms_subproblem_size(ctx.num_buckets, kv);
+    ctx.num_sub_prob = ctx.size_sub_prob ? (n + ctx.size_sub_prob - 1) / ctx.size_sub_prob : 0;
+}
+}  // namespace gms
+
+// src/api/multisplit.cuh:50-61.
+inline void multisplit_allocate_key_only(uint32_t num_elements, multisplit_context& context) {
+    gms::ms_allocate(num_elements, context, 0, "multisplit_allocate_key_only");
+}
+// src/api/multisplit.cuh:63-74.
+inline void multisplit_allocate_key_value(uint32_t num_elements, multisplit_context& context) {
+    gms::ms_allocate(num_elements, context, 1, "multisplit_allocate_key_value");
+}
+// src/api/multisplit.cuh:76-79.
+inline void multisplit_release_memory(multisplit_context& context) {
+    gms::check(gms_multisplit_release(context.handle), "multisplit_release_memory");
+}
+
+// src/api/multisplit.cuh:81-197.  h_offsets (host, B entries) makes the call blocking.
+template <typename KeyT, typename bucket_t>
+inline void multisplit_key_only(KeyT* d_key_in, KeyT* d_key_out, uint32_t num_elements,
+                                multisplit_context& context, bucket_t bucket_identifier,
+                                uint32_t* h_offsets = nullptr, void* stream = nullptr) {
+    gms::require_32bit<KeyT>("key");
+    gms::check(gms_multisplit_key_only(context.handle, reinterpret_cast<const uint32_t*>(d_key_in),
+                                       reinterpret_cast<uint32_t*>(d_key_out), num_elements,
+                                       gms::desc_of(bucket_identifier, context.num_buckets),
+                                       h_offsets, stream),
+               "multisplit_key_only");
+}
+
+// src/api/multisplit.cuh:199-313.
+template <typename KeyT, typename ValueT, typename bucket_t>
+inline void multisplit_key_value(KeyT* d_key_in, ValueT* d_value_in, KeyT* d_key_out,
+                                 ValueT* d_value_out, uint32_t num_elements,
+                                 multisplit_context& context, bucket_t bucket_identifier,
+                                 uint32_t* h_offsets = nullptr, void* stream = nullptr) {
+    gms::require_32bit<KeyT>("key");
+    gms::require_32bit<ValueT>("value");
+    gms::check(gms_multisplit_key_value(
+                   context.handle, reinterpret_cast<const uint32_t*>(d_key_in),
+                   reinterpret_cast<const uint32_t*>(d_value_in),
+                   reinterpret_cast<uint32_t*>(d_key_out), reinterpret_cast<uint32_t*>(d_value_out),
+                   num_elements, gms::desc_of(bucket_identifier, context.num_buckets), h_offsets,
+                   stream),
+               "multisplit_key_value");
+}
+
+// subproblem_size_bms_key_only / _key_value (src/api/bms_api.h:23-89).
+inline uint32_t subproblem_size_bms_key_only(uint32_t num_buckets) {
+    return gms_subproblem_size(num_buckets, 0);
+}
+inline uint32_t subproblem_size_bms_key_value(uint32_t num_buckets) {
+    return gms_subproblem_size(num_buckets, 1);
+}
+
+// ---------------------------------------------------------------- radix sort
+// src/api/multisplit_sort.cuh:100-116.
+class ms_sort_context {
+  public:
+    gms_sort_ctx* handle = nullptr;
+    ms_sort_context() { gms::check(gms_sort_create(&handle), "ms_sort_context"); }
+    ~ms_sort_context() { gms_sort_destroy(handle); }
+    ms_sort_context(const ms_sort_context&) = delete;
+    ms_sort_context& operator=(const ms_sort_context&) = delete;
+};
+
+// src/api/multisplit_sort.cuh:312-318.
+inline void multisplit_sort_release_memory(ms_sort_context& context) {
+    gms::check(gms_sort_release(context.handle), "multisplit_sort_release_memory");
+}
+
+// multisplit_sort_Nbit_allocate[_pairs] (multisplit_sort.cuh:118-309),
+// multisplit_sort_Nbit (:320-559), multisplit_sort_Nbit_pairs
+// (multisplit_sort_pairs.cuh:21-260): keys in, keys out, values in, values out.
+#define GMS_SORT_API(N)                                                                          \
+    inline void multisplit_sort_##N##bit_allocate(uint32_t n_elements, ms_sort_context& ctx) {  \
+        gms::check(gms_sort_allocate(ctx.handle, n_elements, N, 0),                              \
+                   "multisplit_sort_" #N "bit_allocate");                                        \
+    }                                                                                            \
+    inline void multisplit_sort_##N##bit_allocate_pairs(uint32_t n_elements,                     \
+                                                        ms_sort_context& ctx) {                  \
+        gms::check(gms_sort_allocate(ctx.handle, n_elements, N, 1),                              \
+                   "multisplit_sort_" #N "bit_allocate_pairs");                                  \
+    }                                                                                            \
+    inline void multisplit_sort_##N##bit(uint32_t* d_key_in, uint32_t* d_key_out,                \
+                                         uint32_t n_elements, ms_sort_context& ctx,              \
+                                         void* stream = nullptr) {                               \
+        gms::check(gms_sort(ctx.handle, d_key_in, d_key_out, nullptr, nullptr, n_elements, N,    \
+                            stream),                                                             \
+                   "multisplit_sort_" #N "bit");                                                 \
+    }                                                                                            \
+    inline void multisplit_sort_##N##bit_pairs(uint32_t* d_key_in, uint32_t* d_key_out,          \
+                                               uint32_t* d_value_in, uint32_t* d_value_out,      \
+                                               uint32_t n_elements, ms_sort_context& ctx,        \
+                                               void* stream = nullptr) {                         \
+        gms::check(gms_sort(ctx.handle, d_key_in, d_key_out, d_value_in, d_value_out,            \
+                            n_elements, N, stream),                                              \
+                   "multisplit_sort_" #N "bit_pairs");                                           \
+    }
+GMS_SORT_API(4)
+GMS_SORT_API(5)
+GMS_SORT_API(6)
+GMS_SORT_API(7)
+GMS_SORT_API(8)
+#undef GMS_SORT_API
+
+// ---------------------------------------------------------------- compaction
+// src/kernels/compaction/multisplit2_compaction.cuh:477-489.
+class compaction_context {
+  public:
+    gms_compaction_ctx* handle = nullptr;
+    compaction_context() { gms::check(gms_compaction_create(&handle), "compaction_context"); }
+    ~compaction_context() { gms_compaction_destroy(handle); }
+    compaction_context(const compaction_context&) = delete;
+    compaction_context& operator=(const compaction_context&) = delete;
+};
+
+// multisplit2_compaction.cuh:491-517.
+inline void compaction_allocate_key_only(uint32_t num_elements, compaction_context& context) {
+    gms::check(gms_compaction_allocate(context.handle, num_elements, 0),
+               "compaction_allocate_key_only");
+}
+inline void compaction_allocate_key_value(uint32_t num_elements, compaction_context& context) {
+    gms::check(gms_compaction_allocate(context.handle, num_elements, 1),
+               "compaction_allocate_key_value");
+}
+// multisplit2_compaction.cuh:519-523.
+inline void compaction_release_memory(compaction_context& context) {
+    gms::check(gms_compaction_release(context.handle), "compaction_release_memory");
+}
+
+// multisplit2_compaction.cuh:525-548: stable, bucket 0 first; returns the
+// bucket-0 count (blocking, as the reference).
+template <typename key_t, typename bucket_t>
+inline unsigned compaction_key_only(key_t* d_key_in, key_t* d_key_out, uint32_t num_elements,
+                                    compaction_context& context, bucket_t bucket_identifier,
+                                    void* stream = nullptr) {
+    gms::require_32bit<key_t>("key");
+    uint32_t count = 0;
+    gms::check(gms_compaction(context.handle, reinterpret_cast<const uint32_t*>(d_key_in), nullptr,
+                              reinterpret_cast<uint32_t*>(d_key_out), nullptr, num_elements,
+                              gms::desc_of(bucket_identifier, 2), &count, nullptr, stream),
+               "compaction_key_only");
+    return count;
+}
+
+// multisplit2_compaction.cuh:550-573.
+template <typename key_t, typename value_t, typename bucket_t>
+inline unsigned compaction_key_value(key_t* d_key_in, value_t* d_value_in, key_t* d_key_out,
+                                     value_t* d_value_out, uint32_t num_elements,
+                                     compaction_context& context, bucket_t bucket_identifier,
+                                     void* stream = nullptr) {
+    gms::require_32bit<key_t>("key");
+    gms::require_32bit<value_t>("value");
+    uint32_t count = 0;
+    gms::check(gms_compaction(context.handle, reinterpret_cast<const uint32_t*>(d_key_in),
+                              reinterpret_cast<const uint32_t*>(d_value_in),
+                              reinterpret_cast<uint32_t*>(d_key_out),
+                              reinterpret_cast<uint32_t*>(d_value_out), num_elements,
+                              gms::desc_of(bucket_identifier, 2), &count, nullptr, stream),
+               "compaction_key_value");
+    return count;
+}
+
+// ---------------------------------------------------------------- histogram
+// multisplit2_histogram_even_* (src/kernels/histogram/multisplit2_histograms.cuh:36-392):
+// float samples, bin = (uint)((x - lower) * inv_delta), added into d_hist[num_bins].
+inline void multisplit2_histogram_even(const float* d_key_in, uint32_t num_elements, int* d_hist,
+                                       uint32_t num_bins, float lower_level, float upper_level,
+                                       float inverse_delta, void* stream = nullptr) {
+    gms::check(gms_histogram_even_f32(d_key_in, num_elements, reinterpret_cast<int32_t*>(d_hist),
+                                      num_bins, lower_level, upper_level, inverse_delta, stream),
+               "multisplit2_histogram_even");
+}
+
+// Integer-bin histogram through any bucket functor with a gms_bucket_desc
+// (the 64/128/256-bin kernels of multisplit2_histograms.cuh take the bin as a
+// bucket id of the key the same way).
+template <typename KeyT, typename bucket_t>
+inline void multisplit_histogram(const KeyT* d_key_in, uint32_t num_elements, uint32_t* d_hist,
+                                 uint32_t num_bins, bucket_t bucket_identifier,
+                                 void* stream = nullptr) {
+    gms::require_32bit<KeyT>("key");
+    gms::check(gms_histogram_u32(reinterpret_cast<const uint32_t*>(d_key_in), num_elements, d_hist,
+                                 num_bins, gms::desc_of(bucket_identifier, num_bins), stream),
+               "multisplit_histogram");
+}
+
+#endif  // GPUMULTISPLIT_HPP_
