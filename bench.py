@@ -42,7 +42,79 @@ def parse():
     p.add_argument("--cpu-reps", type=int, default=5)
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run one plain copy of n u32 before warm-up (HBM counter calibration)")
+    p.add_argument("--no-batched", action="store_true", help="skip the batched-form measurement")
+    p.add_argument("--batch-reps", type=int, default=5)
     return p.parse_args()
+
+
+def bench_batched(args, torch, gms, W, dev):
+    """Batched form: 4096 independent key-value multisplits, sizes round(2^U),
+    U ~ U[8, 22], 64 buckets by range, uniform keys, all in HBM.  Times the
+    autotuned batch replayed from a hipGraph against the same 4096 instances
+    as back-to-back plain calls on one stream."""
+    import numpy as np
+    count, B = 4096, 64
+    sizes = W.log_uniform_sizes(count, 8, 22, seed=4096)
+    total = int(sizes.sum())
+    print(f"[bench] batched: {count} instances, {total} keys", file=sys.stderr, flush=True)
+    g = torch.Generator(device=dev)
+    g.manual_seed(4096)
+    kin = torch.randint(-(1 << 31), 1 << 31, (total,), dtype=torch.int32, device=dev, generator=g)
+    vin = torch.arange(total, dtype=torch.int32, device=dev)
+    kout = torch.empty_like(kin)
+    vout = torch.empty_like(kin)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    f = gms.range_bucket(B)
+    items = [gms.BatchItem(kin[s:s + n], kout[s:s + n], int(n), B, f, d_value_in=vin[s:s + n],
+                           d_value_out=vout[s:s + n]) for s, n in zip(starts, sizes)]
+    batch = gms.Batch(items)
+    tuned = batch.autotune(reps=2)
+    batch.capture()
+    batch.replay()
+    torch.cuda.synchronize()
+    reps = args.batch_reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.replay()
+    torch.cuda.synchronize()
+    ms_batch = (time.perf_counter() - t0) * 1e3 / reps
+
+    ctx = gms.multisplit_context(B)
+    gms.multisplit_allocate_key_value(int(sizes.max()), ctx)
+    L = gms.lib()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    desc = f.desc()
+    calls = [(kin[s:s + n].data_ptr(), vin[s:s + n].data_ptr(), kout[s:s + n].data_ptr(),
+              vout[s:s + n].data_ptr(), int(n)) for s, n in zip(starts, sizes)]
+
+    def plain_all():
+        for kp, vp, kop, vop, n in calls:
+            st = L.gms_multisplit(ctx.handle, kp, vp, kop, vop, n, desc, None, stream)
+            if st != 0:
+                raise gms.GmsError(st, "gms_multisplit")
+
+    plain_all()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain_all()
+    torch.cuda.synchronize()
+    ms_plain = (time.perf_counter() - t0) * 1e3 / reps
+    gms.multisplit_release_memory(ctx)
+    st = batch.stats()
+    del batch, items
+    return {
+        "workload": f"{count} key-value multisplits, sizes round(2^U) U~U[8,22], {B} buckets (range)",
+        "total_keys": total, "instances": count,
+        "ms_per_batch": round(ms_batch, 4), "gkeys_per_s": round(total / ms_batch / 1e6, 3),
+        "plain_back_to_back_ms": round(ms_plain, 4),
+        "plain_back_to_back_gkeys_per_s": round(total / ms_plain / 1e6, 3),
+        "speedup_vs_plain": round(ms_plain / ms_batch, 3),
+        "hbm_fraction": round(total * 20 / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "launches_per_batch": st["launches"], "work_items": st["work_items"],
+        "tuning": {"items_per_thread": tuned["items_per_thread"], "persistent": tuned["persistent"]},
+        "hipgraph": True, "reps": reps,
+    }
 
 
 def traffic_from_profiles(workload):
@@ -195,6 +267,9 @@ def main():
         "cpu_baseline": cpu,
     }
     gms.multisplit_release_memory(ctx)
+    del kin, vin, kout, vout
+    if not args.no_batched:
+        out["batched"] = bench_batched(args, torch, gms, W, dev)
     print(json.dumps(out), flush=True)
 
 

@@ -52,6 +52,17 @@ class gms_bucket_desc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("param", ctypes.c_uint32), ("width", ctypes.c_uint32)]
 
 
+class gms_batch_item(ctypes.Structure):
+    _fields_ = [("d_key_in", ctypes.c_void_p), ("d_value_in", ctypes.c_void_p),
+                ("d_key_out", ctypes.c_void_p), ("d_value_out", ctypes.c_void_p),
+                ("n", ctypes.c_uint32), ("num_buckets", ctypes.c_uint32),
+                ("desc", gms_bucket_desc), ("d_offsets", ctypes.c_void_p)]
+
+
+class gms_batch_tuning(ctypes.Structure):
+    _fields_ = [("items_per_thread", ctypes.c_uint32), ("persistent", ctypes.c_uint32)]
+
+
 _lib = None
 
 # (name, restype, argtypes) of every exported symbol, in header order.
@@ -90,6 +101,18 @@ _SIGS = [
     ("gms_histogram_u32", _I, [_P, _U32, _P, _U32, gms_bucket_desc, _P]),
     ("gms_histogram_even_f32", _I, [_P, _U32, _P, _U32, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_float, _P]),
+    ("gms_batch_create", _I, [_PP]),
+    ("gms_batch_destroy", _I, [_P]),
+    ("gms_batch_plan", _I, [_P, ctypes.POINTER(gms_batch_item), _U32, _P]),
+    ("gms_batch_run", _I, [_P, _P]),
+    ("gms_batch_capture", _I, [_P]),
+    ("gms_batch_launch_graph", _I, [_P, _P]),
+    ("gms_batch_autotune", _I, [_P, _U32, _P, ctypes.POINTER(gms_batch_tuning),
+                                ctypes.POINTER(ctypes.c_float)]),
+    ("gms_batch_set_tuning", _I, [_P, gms_batch_tuning]),
+    ("gms_batch_get_tuning", _I, [_P, ctypes.POINTER(gms_batch_tuning)]),
+    ("gms_batch_stats", _I, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_U32),
+                             ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_U32)]),
     ("gms_copy_u32", _I, [_P, _P, ctypes.c_size_t, _P]),
 ]
 EXPORTED_SYMBOLS = [s[0] for s in _SIGS]
@@ -545,3 +568,100 @@ def multisplit2_histogram_even(d_key_in, num_elements: int, d_histogram, lower_l
 
 def copy_u32(d_src, d_dst, n: int) -> None:
     _check(lib().gms_copy_u32(_ptr(d_src), _ptr(d_dst), int(n), _stream(d_src)), "copy_u32")
+
+
+# ---------------------------------------------------------------------------
+# Batched multisplit (no reference counterpart): many independent
+# multisplits, planned once, run with a fixed number of launches, replayable
+# through a hipGraph, tile shape picked by the autotuner.
+# ---------------------------------------------------------------------------
+class BatchItem:
+    """One multisplit of a batch (device buffers as torch tensors or pointers)."""
+
+    def __init__(self, d_key_in, d_key_out, num_elements: int, num_buckets: int,
+                 bucket_identifier=None, d_value_in=None, d_value_out=None, d_offsets=None):
+        self.d_key_in, self.d_key_out = d_key_in, d_key_out
+        self.d_value_in, self.d_value_out = d_value_in, d_value_out
+        self.n = int(num_elements)
+        self.num_buckets = int(num_buckets)
+        self.f = bucket_identifier if bucket_identifier is not None else range_bucket(num_buckets)
+        self.d_offsets = d_offsets
+        for name in ("d_key_in", "d_key_out", "d_value_in", "d_value_out"):
+            _check_len(name, getattr(self, name), self.n)
+        _check_len("d_offsets", d_offsets, self.num_buckets)
+
+    def c_item(self) -> gms_batch_item:
+        return gms_batch_item(_ptr(self.d_key_in), _ptr(self.d_value_in), _ptr(self.d_key_out),
+                              _ptr(self.d_value_out), self.n, self.num_buckets, self.f.desc(),
+                              _ptr(self.d_offsets))
+
+
+class Batch:
+    """A planned batch of independent multisplits.
+
+    ``run()`` launches it; ``capture()`` records the launches into a hipGraph
+    that ``replay()`` relaunches; ``autotune()`` picks the tile shape and grid.
+    The items' tensors are kept alive by this object.
+    """
+
+    def __init__(self, items, stream=None):
+        h = ctypes.c_void_p()
+        _check(lib().gms_batch_create(ctypes.byref(h)), "gms_batch_create")
+        self._h = h
+        self.items = list(items)
+        arr = (gms_batch_item * max(len(self.items), 1))(*[it.c_item() for it in self.items])
+        self._stream = stream
+        _check(lib().gms_batch_plan(self._h, arr, len(self.items), self._s()), "gms_batch_plan")
+
+    def _s(self):
+        if self._stream is not None:
+            return self._stream
+        for it in self.items:
+            s = _stream(it.d_key_in, it.d_key_out)
+            if s is not None:
+                return s
+        return None
+
+    def run(self) -> None:
+        _check(lib().gms_batch_run(self._h, self._s()), "gms_batch_run")
+
+    def capture(self) -> None:
+        _check(lib().gms_batch_capture(self._h), "gms_batch_capture")
+
+    def replay(self) -> None:
+        _check(lib().gms_batch_launch_graph(self._h, self._s()), "gms_batch_launch_graph")
+
+    def autotune(self, reps: int = 3):
+        t = gms_batch_tuning()
+        ms = ctypes.c_float()
+        _check(lib().gms_batch_autotune(self._h, int(reps), self._s(), ctypes.byref(t),
+                                        ctypes.byref(ms)), "gms_batch_autotune")
+        return {"items_per_thread": t.items_per_thread, "persistent": t.persistent,
+                "ms_per_run": float(ms.value)}
+
+    @property
+    def tuning(self):
+        t = gms_batch_tuning()
+        _check(lib().gms_batch_get_tuning(self._h, ctypes.byref(t)), "gms_batch_get_tuning")
+        return {"items_per_thread": t.items_per_thread, "persistent": t.persistent}
+
+    @tuning.setter
+    def tuning(self, value):
+        t = gms_batch_tuning(int(value["items_per_thread"]), int(value["persistent"]))
+        _check(lib().gms_batch_set_tuning(self._h, t), "gms_batch_set_tuning")
+
+    def stats(self):
+        k, m = ctypes.c_uint64(), ctypes.c_uint64()
+        w, l = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().gms_batch_stats(self._h, ctypes.byref(k), ctypes.byref(w), ctypes.byref(m),
+                                     ctypes.byref(l)), "gms_batch_stats")
+        return {"keys": k.value, "work_items": w.value, "matrix_words": m.value,
+                "launches": l.value}
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.gms_batch_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass

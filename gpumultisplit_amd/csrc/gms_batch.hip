@@ -1,0 +1,433 @@
+// Batched multisplit: host engine and the gms_batch_* entry points.
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+
+#include "gms_batch.hpp"
+
+namespace gms {
+
+// ---------------------------------------------------------------- buffers
+template <class T>
+gms_status DevVec<T>::reserve(size_t n, hipStream_t s) {
+    if (n <= cap) return GMS_SUCCESS;
+    const size_t want = std::max<size_t>(n + n / 2, 64);
+    if (p) {
+        if (hipStreamSynchronize(s) != hipSuccess) return GMS_ERR_HIP;
+        (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    if (hipMalloc((void**)&p, want * sizeof(T)) != hipSuccess) {
+        p = nullptr;
+        return GMS_ERR_ALLOC;
+    }
+    cap = want;
+    return GMS_SUCCESS;
+}
+
+template <class T>
+void DevVec<T>::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+template <class T>
+gms_status PinnedVec<T>::reserve(size_t n) {
+    if (n <= cap) return GMS_SUCCESS;
+    const size_t want = std::max<size_t>(n + n / 2, 64);
+    T* q = nullptr;
+    if (hipHostMalloc((void**)&q, want * sizeof(T), hipHostMallocDefault) != hipSuccess)
+        return GMS_ERR_ALLOC;
+    if (p) {
+        memcpy(q, p, size * sizeof(T));
+        (void)hipHostFree(p);
+    }
+    p = q;
+    cap = want;
+    return GMS_SUCCESS;
+}
+
+template <class T>
+void PinnedVec<T>::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = size = 0;
+}
+
+template struct DevVec<BatchInst>;
+template struct DevVec<BatchWork>;
+template struct DevVec<uint32_t>;
+template struct PinnedVec<BatchInst>;
+template struct PinnedVec<BatchWork>;
+
+// ---------------------------------------------------------------- checks
+gms_status check_batch_item(const gms_batch_item& it) {
+    if (it.num_buckets < 1 || it.num_buckets > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    gms_status st = check_desc(it.num_buckets, it.desc);
+    if (st != GMS_SUCCESS) return st;
+    if ((it.d_value_in == nullptr) != (it.d_value_out == nullptr)) return GMS_ERR_INVALID_ARG;
+    if (it.n && (!it.d_key_in || !it.d_key_out)) return GMS_ERR_INVALID_ARG;
+    return GMS_SUCCESS;
+}
+
+gms_status check_batch_overlaps(const gms_batch_item* items, uint32_t count) {
+    // Every range of the batch, sorted by start; an output may overlap nothing.
+    struct R {
+        uintptr_t a, b;
+        bool out;
+    };
+    std::vector<R> r;
+    r.reserve((size_t)count * 5);
+    for (uint32_t i = 0; i < count; ++i) {
+        const gms_batch_item& it = items[i];
+        const size_t bytes = (size_t)it.n * 4;
+        if (bytes == 0) continue;
+        r.push_back({(uintptr_t)it.d_key_in, (uintptr_t)it.d_key_in + bytes, false});
+        r.push_back({(uintptr_t)it.d_key_out, (uintptr_t)it.d_key_out + bytes, true});
+        if (it.d_value_in) {
+            r.push_back({(uintptr_t)it.d_value_in, (uintptr_t)it.d_value_in + bytes, false});
+            r.push_back({(uintptr_t)it.d_value_out, (uintptr_t)it.d_value_out + bytes, true});
+        }
+        if (it.d_offsets) {
+            const uintptr_t o = (uintptr_t)it.d_offsets;
+            r.push_back({o, o + (size_t)it.num_buckets * 4, true});
+        }
+    }
+    std::sort(r.begin(), r.end(), [](const R& x, const R& y) { return x.a < y.a; });
+    // Sweep: track the furthest end so far, and the furthest end of an output.
+    uintptr_t end_any = 0, end_out = 0;
+    for (const R& x : r) {
+        if (x.out ? x.a < end_any : x.a < end_out) return GMS_ERR_ALIAS;
+        end_any = std::max(end_any, x.b);
+        if (x.out) end_out = std::max(end_out, x.b);
+    }
+    return GMS_SUCCESS;
+}
+
+// ---------------------------------------------------------------- engine
+static int class_of(uint32_t num_buckets) {
+    const int l = ceil_log2_u32(num_buckets);
+    return l < 1 ? 1 : l;
+}
+
+gms_status BatchEngine::plan(const gms_batch_item* items, uint32_t count, int slot) {
+    const uint32_t K = tuning.items_per_thread;
+    if (K != 8 && K != 16 && K != 24) return GMS_ERR_INVALID_ARG;
+    const uint32_t tile = (uint32_t)kThreads * K;
+    slot_ = slot & 1;
+    PinnedVec<BatchInst>& hi = h_inst_[slot_];
+    PinnedVec<BatchWork>& hw = h_work_[slot_];
+
+    // Group order: (class, kv); instances keep their order inside a group.
+    std::vector<uint32_t> order(count);
+    for (uint32_t i = 0; i < count; ++i) order[i] = i;
+    auto key = [&](uint32_t i) {
+        return class_of(items[i].num_buckets) * 2 + (items[i].d_value_in ? 1 : 0);
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return key(a) < key(b); });
+
+    uint64_t tiles = 0;
+    for (uint32_t i = 0; i < count; ++i)
+        tiles += items[i].n ? ((uint64_t)items[i].n + tile - 1) / tile : 1;
+    if (tiles >= (1ull << 31)) return GMS_ERR_CAPACITY;
+    gms_status st = hi.reserve(count);
+    if (st == GMS_SUCCESS) st = hw.reserve((size_t)tiles);
+    if (st != GMS_SUCCESS) return st;
+
+    groups.clear();
+    uint64_t mbase = 0, obase = 0;
+    uint32_t w = 0;
+    for (uint32_t j = 0; j < count; ++j) {
+        const gms_batch_item& it = items[order[j]];
+        const int lb = class_of(it.num_buckets);
+        const bool kv = it.d_value_in != nullptr;
+        if (groups.empty() || groups.back().log_b != lb || groups.back().kv != kv)
+            groups.push_back(BatchGroup{lb, kv, w, w});
+        const uint32_t nt = it.n ? (uint32_t)(((uint64_t)it.n + tile - 1) / tile) : 1u;
+        BatchInst& I = hi.p[j];
+        I.kin = it.d_key_in;
+        I.vin = it.d_value_in;
+        I.kout = it.d_key_out;
+        I.vout = it.d_value_out;
+        I.offs = it.d_offsets;
+        I.n = it.n;
+        I.num_tiles = nt;
+        I.mbase = (uint32_t)mbase;
+        I.obase = (uint32_t)obase;
+        I.nb = it.num_buckets;
+        const bool al = (((uintptr_t)it.d_key_in) & 15u) == 0 &&
+                        (((uintptr_t)it.d_value_in) & 15u) == 0;
+        I.vec = al ? 1u : 0u;
+        I.f = AnyF::make(it.desc.kind, it.desc.param, it.desc.width, it.num_buckets);
+        for (uint32_t t = 0; t < nt; ++t) hw.p[w++] = BatchWork{j, t};
+        groups.back().w1 = w;
+        mbase += ((uint64_t)1 << lb) * nt;
+        obase += it.n;
+        if (mbase >= 0xFFFFFFFFull || obase >= 0xFFFFFFFFull) return GMS_ERR_CAPACITY;
+    }
+    hi.size = count;
+    hw.size = w;
+    num_instances = count;
+    num_work = w;
+    m_total = mbase;
+    keys_total = obase;
+    return GMS_SUCCESS;
+}
+
+gms_status BatchEngine::upload(hipStream_t s) {
+    if (num_cus_ == 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return GMS_ERR_NO_DEVICE;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return GMS_ERR_HIP;
+        num_cus_ = cus > 0 ? cus : 1;
+    }
+    gms_status st = d_inst_.reserve(std::max<uint32_t>(num_instances, 1), s);
+    if (st == GMS_SUCCESS) st = d_work_.reserve(std::max<uint32_t>(num_work, 1), s);
+    if (st == GMS_SUCCESS) st = d_mat_.reserve(std::max<uint64_t>(m_total, 4), s);
+    if (st == GMS_SUCCESS) st = d_part_.reserve(scan_partials_needed(std::max<uint64_t>(m_total, 4)), s);
+    if (st == GMS_SUCCESS) st = d_queue_.reserve(2 * std::max<size_t>(groups.size(), 1), s);
+    if (st != GMS_SUCCESS) return st;
+    if (num_instances &&
+        hipMemcpyAsync(d_inst_.p, h_inst_[slot_].p, (size_t)num_instances * sizeof(BatchInst),
+                       hipMemcpyHostToDevice, s) != hipSuccess)
+        return GMS_ERR_HIP;
+    if (num_work &&
+        hipMemcpyAsync(d_work_.p, h_work_[slot_].p, (size_t)num_work * sizeof(BatchWork),
+                       hipMemcpyHostToDevice, s) != hipSuccess)
+        return GMS_ERR_HIP;
+    return GMS_SUCCESS;
+}
+
+uint32_t BatchEngine::grid_for(const BatchGroup& g, bool post) const {
+    const uint32_t nw = g.w1 - g.w0;
+    if (!tuning.persistent) return nw;
+    int occ = 1;
+    switch (tuning.items_per_thread) {
+        case 8: occ = batch_occupancy_k8(g.log_b, g.kv, post); break;
+        case 16: occ = batch_occupancy_k16(g.log_b, g.kv, post); break;
+        default: occ = batch_occupancy_k24(g.log_b, g.kv, post); break;
+    }
+    const uint64_t resident = (uint64_t)occ * (uint64_t)num_cus_;
+    return (uint32_t)std::min<uint64_t>(nw, resident);
+}
+
+uint32_t BatchEngine::launches() const {
+    if (num_work == 0) return 0;
+    return 1u + 2u * (uint32_t)groups.size() + (m_total ? 3u : 0u);
+}
+
+gms_status BatchEngine::launch(hipStream_t s) {
+    if (num_work == 0) return GMS_SUCCESS;
+    const uint32_t G = (uint32_t)groups.size();
+    hipError_t e = launch_fill(d_queue_.p, 0u, 2 * G, s);
+    auto one = [&](const BatchGroup& g, uint32_t gi, bool post) {
+        BatchLaunch c{d_inst_.p, d_work_.p + g.w0, g.w1 - g.w0, grid_for(g, post), d_mat_.p,
+                      (uint32_t)m_total, (uint32_t)keys_total, d_queue_.p + 2 * gi + (post ? 1 : 0),
+                      g.log_b, g.kv, s};
+        switch (tuning.items_per_thread) {
+            case 8: return post ? launch_batch_postscan_k8(c) : launch_batch_prescan_k8(c);
+            case 16: return post ? launch_batch_postscan_k16(c) : launch_batch_prescan_k16(c);
+            default: return post ? launch_batch_postscan_k24(c) : launch_batch_prescan_k24(c);
+        }
+    };
+    for (uint32_t gi = 0; gi < G && e == hipSuccess; ++gi) e = one(groups[gi], gi, false);
+    if (e == hipSuccess)
+        e = launch_exclusive_scan(d_mat_.p, d_mat_.p, (uint32_t)m_total, d_part_.p, s);
+    for (uint32_t gi = 0; gi < G && e == hipSuccess; ++gi) e = one(groups[gi], gi, true);
+    return hip_status(e);
+}
+
+void BatchEngine::release() {
+    for (int i = 0; i < 2; ++i) {
+        h_inst_[i].release();
+        h_work_[i].release();
+    }
+    d_inst_.release();
+    d_work_.release();
+    d_mat_.release();
+    d_part_.release();
+    d_queue_.release();
+    num_work = num_instances = 0;
+    m_total = keys_total = 0;
+    groups.clear();
+}
+
+}  // namespace gms
+
+using namespace gms;
+
+struct gms_batch {
+    BatchEngine eng;
+    std::vector<gms_batch_item> items;
+    bool planned = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t cap_stream = nullptr;
+
+    void drop_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    gms_status replan(hipStream_t s) {
+        drop_graph();
+        planned = false;
+        gms_status st = eng.plan(items.data(), (uint32_t)items.size(), 0);
+        if (st == GMS_SUCCESS) st = eng.upload(s);
+        if (st == GMS_SUCCESS) st = hip_status(hipStreamSynchronize(s));
+        planned = st == GMS_SUCCESS;
+        return st;
+    }
+};
+
+extern "C" {
+
+gms_status gms_batch_create(gms_batch** out) {
+    if (!out) return GMS_ERR_INVALID_ARG;
+    *out = new (std::nothrow) gms_batch();
+    return *out ? GMS_SUCCESS : GMS_ERR_ALLOC;
+}
+
+gms_status gms_batch_destroy(gms_batch* b) {
+    if (!b) return GMS_SUCCESS;
+    b->drop_graph();
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    b->eng.release();
+    delete b;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_batch_plan(gms_batch* b, const gms_batch_item* items, uint32_t count,
+                          void* stream) {
+    if (!b || (count && !items)) return GMS_ERR_INVALID_ARG;
+    for (uint32_t i = 0; i < count; ++i) {
+        gms_status st = check_batch_item(items[i]);
+        if (st != GMS_SUCCESS) return st;
+    }
+    gms_status st = check_batch_overlaps(items, count);
+    if (st != GMS_SUCCESS) return st;
+    b->items.assign(items, items + count);
+    return b->replan((hipStream_t)stream);
+}
+
+gms_status gms_batch_run(gms_batch* b, void* stream) {
+    if (!b) return GMS_ERR_INVALID_ARG;
+    if (!b->planned) return GMS_ERR_NOT_ALLOCATED;
+    return b->eng.launch((hipStream_t)stream);
+}
+
+gms_status gms_batch_capture(gms_batch* b) {
+    if (!b) return GMS_ERR_INVALID_ARG;
+    if (!b->planned) return GMS_ERR_NOT_ALLOCATED;
+    b->drop_graph();
+    if (!b->cap_stream && hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess) {
+        b->cap_stream = nullptr;
+        return GMS_ERR_HIP;
+    }
+    if (hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return GMS_ERR_HIP;
+    gms_status st = b->eng.launch(b->cap_stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+    if (st != GMS_SUCCESS || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return st != GMS_SUCCESS ? st : GMS_ERR_HIP;
+    }
+    b->graph = g;
+    if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        b->exec = nullptr;
+        b->drop_graph();
+        return GMS_ERR_HIP;
+    }
+    return GMS_SUCCESS;
+}
+
+gms_status gms_batch_launch_graph(gms_batch* b, void* stream) {
+    if (!b) return GMS_ERR_INVALID_ARG;
+    if (!b->exec) return GMS_ERR_NOT_ALLOCATED;
+    return hip_status(hipGraphLaunch(b->exec, (hipStream_t)stream));
+}
+
+gms_status gms_batch_set_tuning(gms_batch* b, gms_batch_tuning t) {
+    if (!b) return GMS_ERR_INVALID_ARG;
+    if (t.items_per_thread != 8 && t.items_per_thread != 16 && t.items_per_thread != 24)
+        return GMS_ERR_INVALID_ARG;
+    if (t.persistent > 1) return GMS_ERR_INVALID_ARG;
+    b->eng.tuning.items_per_thread = t.items_per_thread;
+    b->eng.tuning.persistent = t.persistent;
+    return b->planned ? b->replan(nullptr) : GMS_SUCCESS;
+}
+
+gms_status gms_batch_get_tuning(const gms_batch* b, gms_batch_tuning* t) {
+    if (!b || !t) return GMS_ERR_INVALID_ARG;
+    t->items_per_thread = b->eng.tuning.items_per_thread;
+    t->persistent = b->eng.tuning.persistent;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_batch_autotune(gms_batch* b, uint32_t reps, void* stream, gms_batch_tuning* chosen,
+                              float* ms_per_run) {
+    if (!b || reps == 0) return GMS_ERR_INVALID_ARG;
+    if (!b->planned) return GMS_ERR_NOT_ALLOCATED;
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return GMS_ERR_HIP;
+    }
+    const BatchTuning keep = b->eng.tuning;
+    BatchTuning best = keep;
+    float best_ms = -1.f;
+    gms_status st = GMS_SUCCESS;
+    for (uint32_t K : {8u, 16u, 24u}) {
+        for (uint32_t pers : {0u, 1u}) {
+            b->eng.tuning.items_per_thread = K;
+            b->eng.tuning.persistent = pers;
+            st = b->replan(s);
+            if (st == GMS_SUCCESS) st = b->eng.launch(s);  // warm-up
+            if (st == GMS_SUCCESS) st = hip_status(hipEventRecord(e0, s));
+            for (uint32_t r = 0; r < reps && st == GMS_SUCCESS; ++r) st = b->eng.launch(s);
+            if (st == GMS_SUCCESS) st = hip_status(hipEventRecord(e1, s));
+            if (st == GMS_SUCCESS) st = hip_status(hipEventSynchronize(e1));
+            float ms = 0.f;
+            if (st == GMS_SUCCESS) st = hip_status(hipEventElapsedTime(&ms, e0, e1));
+            if (st != GMS_SUCCESS) break;
+            ms /= (float)reps;
+            if (best_ms < 0.f || ms < best_ms) {
+                best_ms = ms;
+                best = b->eng.tuning;
+            }
+        }
+        if (st != GMS_SUCCESS) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->eng.tuning = st == GMS_SUCCESS ? best : keep;
+    const gms_status st2 = b->replan(s);
+    if (st != GMS_SUCCESS) return st;
+    if (st2 != GMS_SUCCESS) return st2;
+    if (chosen) {
+        chosen->items_per_thread = best.items_per_thread;
+        chosen->persistent = best.persistent;
+    }
+    if (ms_per_run) *ms_per_run = best_ms;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_batch_stats(const gms_batch* b, uint64_t* keys, uint32_t* work_items,
+                           uint64_t* matrix_words, uint32_t* launches) {
+    if (!b) return GMS_ERR_INVALID_ARG;
+    if (keys) *keys = b->eng.keys_total;
+    if (work_items) *work_items = b->eng.num_work;
+    if (matrix_words) *matrix_words = b->eng.m_total;
+    if (launches) *launches = b->eng.launches();
+    return GMS_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// Host engine of the batched multisplit: plans a list of instances into
+// device tables (instances, (instance, tile) work items, the concatenated
+// histogram matrix) and launches the batched kernels on a stream.  Used by
+// the gms_batch_* entry points (plan once, replay through a hipGraph) and by
+// the request server (a new plan every step, tables staged through pinned
+// host memory so that planning overlaps the previous step on the GPU).
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "gms_batch_kernels.hpp"
+#include "gms_internal.hpp"
+
+namespace gms {
+
+// Device buffer that only grows (the stream is synchronised before a grow).
+template <class T>
+struct DevVec {
+    T* p = nullptr;
+    size_t cap = 0;
+    gms_status reserve(size_t n, hipStream_t s);
+    void release();
+};
+
+// Pinned host buffer that only grows.
+template <class T>
+struct PinnedVec {
+    T* p = nullptr;
+    size_t cap = 0;
+    size_t size = 0;
+    gms_status reserve(size_t n);
+    void release();
+};
+
+struct BatchGroup {
+    int log_b;
+    bool kv;
+    uint32_t w0, w1;  // work items [w0, w1)
+};
+
+struct BatchTuning {
+    uint32_t items_per_thread = 24;  // 8, 16 or 24 keys per thread (tile = 256 x this)
+    uint32_t persistent = 0;         // 0: one workgroup per tile; 1: resident grid + work queue
+};
+
+class BatchEngine {
+  public:
+    BatchTuning tuning;
+    // Plan `count` instances (validated).  Host tables go to staging slot
+    // `slot` (0 or 1): the caller makes sure no copy from that slot is still
+    // pending.  Tables are copied to the device on `stream` by upload().
+    gms_status plan(const gms_batch_item* items, uint32_t count, int slot);
+    gms_status upload(hipStream_t stream);
+    // Queue reset, prescans, one scan, postscans -- all kernels, no memsets.
+    gms_status launch(hipStream_t stream);
+    void release();
+
+    uint64_t keys_total = 0;
+    uint64_t m_total = 0;
+    uint32_t num_work = 0;
+    uint32_t num_instances = 0;
+    std::vector<BatchGroup> groups;
+    uint32_t launches() const;
+
+  private:
+    int slot_ = 0;
+    PinnedVec<BatchInst> h_inst_[2];
+    PinnedVec<BatchWork> h_work_[2];
+    DevVec<BatchInst> d_inst_;
+    DevVec<BatchWork> d_work_;
+    DevVec<uint32_t> d_mat_;
+    DevVec<uint32_t> d_part_;
+    DevVec<uint32_t> d_queue_;
+    int num_cus_ = 0;
+    uint32_t grid_for(const BatchGroup& g, bool post) const;
+};
+
+// Checks of one item (bucket count, descriptor, pointers, in/out overlap).
+gms_status check_batch_item(const gms_batch_item& it);
+// Checks that no output range overlaps any other range of the batch.
+gms_status check_batch_overlaps(const gms_batch_item* items, uint32_t count);
+
+}  // namespace gms

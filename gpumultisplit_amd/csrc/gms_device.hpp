@@ -7,11 +7,14 @@
 #include "hip_emu.hpp"
 #define GMS_UPDATE_DPP gms_emu::update_dpp
 #define GMS_READLANE gms_emu::readlane
+#define GMS_READFIRSTLANE(x) (x)
 #define GMS_WAVE_SYNC() gms_emu::wave_sync()
 #else
 #include <hip/hip_runtime.h>
 #define GMS_UPDATE_DPP __builtin_amdgcn_update_dpp
 #define GMS_READLANE __builtin_amdgcn_readlane
+// A value the whole wave holds alike, moved to a scalar register.
+#define GMS_READFIRSTLANE(x) __builtin_amdgcn_readfirstlane(x)
 // Orders the LDS accesses of one wave before and after this point across its
 // lanes.  The lanes of a wave execute in lockstep, so this only stops the
 // compiler from moving memory operations across it; it emits no instruction.
@@ -79,6 +82,46 @@ struct DigitF {
 struct IdentF {
     uint32_t bmax;
     GMS_HD uint32_t operator()(uint32_t k) const { return k < bmax ? k : bmax; }
+};
+
+// Any of the identifiers above, chosen at run time: the batched kernels give
+// every instance of a batch its own.  One branch-free formula covers them all:
+//   t = mulhi(k, m);  q = (t + (((k - t) >> sh1) & em)) >> sh2;
+//   id = min(q & mask, bmax)
+// range:    m = B, em = 0                       -> q = floor(k*B / 2^32)
+// delta:    m, sh1, sh2 of DeltaF, em = ~0      -> q = floor(k / d)
+// digit:    m = 0, em = ~0, sh1 = 0, sh2 = shift -> q = k >> shift
+// identity: m = 0, em = ~0, sh1 = sh2 = 0       -> q = k
+struct AnyF {
+    uint32_t m, em, sh1, sh2, mask, bmax;
+    GMS_HD uint32_t operator()(uint32_t k) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        const uint32_t t = __umulhi(k, m);
+#else
+        const uint32_t t = (uint32_t)(((uint64_t)k * m) >> 32);
+#endif
+        const uint32_t q = ((t + (((k - t) >> sh1) & em)) >> sh2) & mask;
+        return q < bmax ? q : bmax;
+    }
+    // kind: 0 range, 1 delta(param), 2 digit(shift = param, width bits;
+    // width 0 = log2 B), 3 identity, 4 low bits(param) -- the gms_bucket_kind
+    // numbering.  Arguments are assumed validated.
+    static AnyF make(uint32_t kind, uint32_t param, uint32_t width, uint32_t num_buckets) {
+        const uint32_t bmax = num_buckets - 1u;
+        switch (kind) {
+            case 0: return AnyF{num_buckets, 0u, 0u, 0u, ~0u, bmax};
+            case 1: {
+                const DeltaF d = DeltaF::make(param, num_buckets);
+                return AnyF{d.m, ~0u, d.sh1, d.sh2, ~0u, bmax};
+            }
+            case 2: {
+                const uint32_t mask = width ? (1u << width) - 1u : bmax;
+                return AnyF{0u, ~0u, 0u, param, mask, bmax};
+            }
+            case 4: return AnyF{0u, ~0u, 0u, 0u, (1u << param) - 1u, bmax};
+            default: return AnyF{0u, ~0u, 0u, 0u, ~0u, bmax};
+        }
+    }
 };
 
 // ---------------------------------------------------------------------------

@@ -220,6 +220,66 @@ gms_status gms_histogram_even_f32(const float *d_samples, uint32_t n, int32_t *d
                                   float inv_delta, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Batched multisplit (no counterpart in the reference)                      */
+/* ------------------------------------------------------------------------ */
+
+/* One independent multisplit of a batch.  Same meaning as the arguments of
+ * gms_multisplit: values NULL = key-only; d_offsets (device, num_buckets
+ * entries) receives the bucket starts or is NULL.  Outputs of different items
+ * must not overlap any other item's input or output. */
+typedef struct gms_batch_item {
+    const uint32_t *d_key_in;
+    const uint32_t *d_value_in;
+    uint32_t *d_key_out;
+    uint32_t *d_value_out;
+    uint32_t n;
+    uint32_t num_buckets;   /* 1..256 */
+    gms_bucket_desc desc;
+    uint32_t *d_offsets;
+} gms_batch_item;
+
+/* Tile shape and grid of the batched kernels (picked by gms_batch_autotune). */
+typedef struct gms_batch_tuning {
+    uint32_t items_per_thread;  /* 8, 16 or 24: tile = 256 x items_per_thread keys */
+    uint32_t persistent;        /* 0: one workgroup per tile; 1: resident grid
+                                   pulling tiles from a device work queue */
+} gms_batch_tuning;
+
+typedef struct gms_batch gms_batch;
+
+gms_status gms_batch_create(gms_batch **out);
+gms_status gms_batch_destroy(gms_batch *b);
+
+/* Validates and plans `count` items: builds the (instance, tile) work list and
+ * the device tables, and sizes the concatenated histogram matrix.  Blocks
+ * until the tables are on the device.  The sum of n and the matrix size must
+ * stay below 2^32.  Drops any captured graph.  Buffers are not touched. */
+gms_status gms_batch_plan(gms_batch *b, const gms_batch_item *items, uint32_t count,
+                          void *stream);
+
+/* Runs every planned multisplit: one prescan and one postscan launch per
+ * (bucket class, key-value) group and one scan, whatever the item count. */
+gms_status gms_batch_run(gms_batch *b, void *stream);
+
+/* Captures the launch sequence of the current plan into a hipGraph (kernel
+ * nodes only) and instantiates it; gms_batch_launch_graph replays it. */
+gms_status gms_batch_capture(gms_batch *b);
+gms_status gms_batch_launch_graph(gms_batch *b, void *stream);
+
+/* Times every tuning candidate on the planned batch (reps runs each, outputs
+ * are overwritten with the same results), keeps the fastest, re-plans with it
+ * and reports it with its time per run in milliseconds. */
+gms_status gms_batch_autotune(gms_batch *b, uint32_t reps, void *stream,
+                              gms_batch_tuning *chosen, float *ms_per_run);
+gms_status gms_batch_set_tuning(gms_batch *b, gms_batch_tuning t);
+gms_status gms_batch_get_tuning(const gms_batch *b, gms_batch_tuning *t);
+
+/* Shape of the current plan: keys, work items (tiles), matrix words, kernel
+ * launches per run. */
+gms_status gms_batch_stats(const gms_batch *b, uint64_t *keys, uint32_t *work_items,
+                           uint64_t *matrix_words, uint32_t *launches);
+
+/* ------------------------------------------------------------------------ */
 /* Utilities                                                                 */
 /* ------------------------------------------------------------------------ */
 

@@ -207,6 +207,50 @@ int main(int argc, char** argv) {
         EXPECT(memcmp(hist.p, wh.data(), 256 * 4) == 0, "histogram n=%u", n);
     }
     printf("compaction/histogram done\n");
+
+    // batched multisplit: exact-size buffers per instance, every tuning
+    for (uint32_t K : {8u, 16u, 24u}) {
+        for (uint32_t pers : {0u, 1u}) {
+            const uint32_t ns[] = {0, 1, 777, 2048, 4096 + 3, 6144, 13000};
+            const uint32_t nbs[] = {8, 2, 1, 100, 64, 256, 17};
+            const int nk = 7;
+            std::vector<Buf*> bufs;
+            std::vector<gms_batch_item> items;
+            for (int i = 0; i < nk; ++i) {
+                const uint32_t n = ns[i], B = nbs[i];
+                const bool kv = i % 2 == 0;
+                Buf *ki = new Buf(n), *vi = new Buf(n), *ko = new Buf(n), *vo = new Buf(n),
+                    *of = new Buf(B);
+                for (uint32_t j = 0; j < n; ++j) {
+                    ki->p[j] = rnd();
+                    vi->p[j] = j;
+                }
+                bufs.insert(bufs.end(), {ki, vi, ko, vo, of});
+                items.push_back(gms_batch_item{ki->p, kv ? vi->p : nullptr, ko->p,
+                                               kv ? vo->p : nullptr, n, B,
+                                               gms_bucket_desc{GMS_BUCKET_RANGE, 0, 0}, of->p});
+            }
+            gms_batch* b;
+            gms_batch_create(&b);
+            EXPECT(gms_batch_set_tuning(b, gms_batch_tuning{K, pers}) == GMS_SUCCESS, "tuning");
+            EXPECT(gms_batch_plan(b, items.data(), nk, nullptr) == GMS_SUCCESS, "batch plan");
+            EXPECT(gms_batch_run(b, nullptr) == GMS_SUCCESS, "batch run");
+            gms_batch_destroy(b);
+            for (int i = 0; i < nk; ++i) {
+                const gms_batch_item& it = items[i];
+                std::vector<uint32_t> wk(it.n + 1), wv(it.n + 1), wo(it.num_buckets);
+                ms_oracle_multisplit(GMS_BUCKET_RANGE, it.num_buckets, 0, it.d_key_in,
+                                     it.d_value_in, wk.data(), it.d_value_in ? wv.data() : nullptr,
+                                     it.n, wo.data());
+                EXPECT(memcmp(it.d_key_out, wk.data(), it.n * 4) == 0 &&
+                           (!it.d_value_in || memcmp(it.d_value_out, wv.data(), it.n * 4) == 0) &&
+                           memcmp(it.d_offsets, wo.data(), it.num_buckets * 4) == 0,
+                       "batch K=%u persistent=%u instance %d", K, pers, i);
+            }
+            for (Buf* x : bufs) delete x;
+        }
+    }
+    printf("batch done\n");
     printf(failures ? "EMU CHECK FAILED (%d)\n" : "EMU CHECK OK\n", failures);
     return failures ? 1 : 0;
 }

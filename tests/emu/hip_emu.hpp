@@ -130,5 +130,51 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
     return hipSuccess;
 }
 
+// Pinned host memory is ordinary host memory here.
+#define hipHostMallocDefault 0u
+inline hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) {
+    *p = aligned_alloc(256, ((bytes + 255) / 256) * 256);
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+inline hipError_t hipHostFree(void* p) {
+    free(p);
+    return hipSuccess;
+}
+
+// Device attributes and occupancy of the emulated device.
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 0 };
+inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) {
+    *v = 4;
+    return hipSuccess;
+}
+template <class Fn>
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, Fn, int, size_t) {
+    *n = 2;
+    return hipSuccess;
+}
+
+// Streams exist only as handles; graphs are not emulated.
+#define hipStreamNonBlocking 1u
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1 };
+typedef struct gms_emu_graph* hipGraph_t;
+typedef struct gms_emu_graph_exec* hipGraphExec_t;
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+    static int dummy;
+    *s = &dummy;
+    return hipSuccess;
+}
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorInvalidValue; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+    *g = nullptr;
+    return hipErrorInvalidValue;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t*, hipGraph_t, void*, void*, size_t) {
+    return hipErrorInvalidValue;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorInvalidValue; }
+inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+
 #define hipLaunchKernelGGL(kern, grid, block, shmem, stream, ...) \
     gms_emu::launch(dim3(grid), dim3(block), [=]() { kern(__VA_ARGS__); })

@@ -1,0 +1,182 @@
+"""GPU parity of the batched multisplit (gms_batch_*): every instance of a
+batch must equal the oracle run on that instance alone, under every tuning,
+through a captured hipGraph replayed several times, and after the autotuner
+has chosen a tuning.  At the batched-workload shape (4096 key-value
+instances, sizes log-uniform in [2^8, 2^22], 64 buckets) every instance is
+compared with the plain multisplit of the same instance.
+"""
+import numpy as np
+import pytest
+
+import gpumultisplit_amd as gms
+import oracle
+from gpumultisplit_amd import workloads as W
+from gpu_util import banner, filled_u32, from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+# (n, B, functor factory, oracle kind, oracle param, kv)
+MIXED = [
+    (0, 8, lambda: gms.range_bucket(8), oracle.RANGE, 0, True),
+    (1, 2, lambda: gms.range_bucket(2), oracle.RANGE, 0, False),
+    (255, 1, lambda: gms.range_bucket(1), oracle.RANGE, 0, True),
+    (6143, 5, lambda: gms.range_bucket(5), oracle.RANGE, 0, True),
+    (6144, 64, lambda: gms.range_bucket(64), oracle.RANGE, 0, True),
+    (6145, 64, lambda: gms.range_bucket(64), oracle.RANGE, 0, False),
+    (100_003, 256, lambda: gms.range_bucket(256), oracle.RANGE, 0, True),
+    (70_000, 32, lambda: gms.delta_func((1 << 32) // 32 + 1), oracle.DELTA, (1 << 32) // 32 + 1, True),
+    (50_000, 7, lambda: gms.delta_func(1000), oracle.DELTA, 1000, True),
+    (65_536, 256, lambda: gms.ms_bucket_sort_8bit(16), oracle.DIGIT, 16, False),
+    (33_333, 128, lambda: gms.ms_bucket_sort_7bit(25), oracle.DIGIT, 25, True),
+    (20_000, 16, lambda: gms.identity_bucket(), oracle.IDENTITY, 0, True),
+    (12_345, 2, lambda: gms.test_bucket(1), oracle.LOWBITS, 1, False),
+    (300_001, 100, lambda: gms.range_bucket(100), oracle.RANGE, 0, True),
+]
+
+
+def _make(dev, cases, seed):
+    items, want = [], []
+    for i, (n, B, ff, kind, param, kv) in enumerate(cases):
+        keys = W.uniform_keys(n, seed=seed + i)
+        if kind == oracle.IDENTITY:
+            keys = keys % np.uint32(B + 5)
+        if kind == oracle.DELTA and param == 1000:
+            keys = keys % np.uint32(9000)
+        vals = (np.arange(n, dtype=np.uint32) * np.uint32(3) + np.uint32(i)).astype(np.uint32)
+        it = gms.BatchItem(to_dev(keys, dev) if n else filled_u32(0, dev), filled_u32(n, dev), n, B,
+                           ff(), d_value_in=(to_dev(vals, dev) if n else filled_u32(0, dev)) if kv else None,
+                           d_value_out=filled_u32(n, dev) if kv else None,
+                           d_offsets=filled_u32(B, dev, 0xFFFFFFFF))
+        items.append(it)
+        want.append(oracle.multisplit(keys, B, kind, param, vals if kv else None))
+    return items, want
+
+
+def _check(items, want, label):
+    for i, (it, (wk, wv, wo)) in enumerate(zip(items, want)):
+        n = it.n
+        np.testing.assert_array_equal(from_dev(it.d_key_out)[:n], wk, err_msg=f"{label} inst {i} keys")
+        np.testing.assert_array_equal(from_dev(it.d_offsets), wo, err_msg=f"{label} inst {i} offsets")
+        if it.d_value_in is not None:
+            np.testing.assert_array_equal(from_dev(it.d_value_out)[:n], wv,
+                                          err_msg=f"{label} inst {i} values")
+
+
+def _clear(items):
+    for it in items:
+        it.d_key_out.fill_(0x5A5A5A5A)
+        it.d_offsets.fill_(-1)
+        if it.d_value_out is not None:
+            it.d_value_out.fill_(0x5A5A5A5A)
+
+
+@pytest.mark.parametrize("tuning", [(8, 0), (8, 1), (16, 0), (16, 1), (24, 0), (24, 1)])
+def test_batch_mixed_matches_oracle(dev, tuning):
+    import torch
+    banner(f"batch mixed tuning={tuning}")
+    items, want = _make(dev, MIXED, seed=17)
+    b = gms.Batch(items)
+    b.tuning = {"items_per_thread": tuning[0], "persistent": tuning[1]}
+    b.run()
+    torch.cuda.synchronize()
+    _check(items, want, f"run {tuning}")
+    # run again on the same plan: same result
+    _clear(items)
+    b.run()
+    torch.cuda.synchronize()
+    _check(items, want, f"rerun {tuning}")
+
+
+def test_batch_graph_replays_match_eager(dev):
+    import torch
+    banner("batch graph replay")
+    items, want = _make(dev, MIXED, seed=99)
+    b = gms.Batch(items)
+    b.capture()
+    for rep in range(4):
+        _clear(items)                      # eager work between replays
+        torch.cuda.synchronize()
+        b.replay()
+        torch.cuda.synchronize()
+        _check(items, want, f"replay {rep}")
+
+
+def test_batch_autotune_keeps_results(dev):
+    import torch
+    banner("batch autotune")
+    items, want = _make(dev, MIXED, seed=5)
+    b = gms.Batch(items)
+    t = b.autotune(reps=2)
+    assert t["items_per_thread"] in (8, 16, 24) and t["persistent"] in (0, 1)
+    assert t["ms_per_run"] > 0
+    assert b.tuning == {"items_per_thread": t["items_per_thread"], "persistent": t["persistent"]}
+    _clear(items)
+    b.run()
+    torch.cuda.synchronize()
+    _check(items, want, "autotuned")
+    st = b.stats()
+    assert st["keys"] == sum(c[0] for c in MIXED)
+    assert st["launches"] == 1 + 3 + 2 * len({(max(1, int(np.ceil(np.log2(c[1])))), c[5]) for c in MIXED})
+
+
+def test_batch_validation(dev):
+    a = filled_u32(64, dev)
+    o = filled_u32(64, dev)
+    with pytest.raises(gms.GmsError):          # 300 buckets
+        gms.Batch([gms.BatchItem(a, o, 64, 300, gms.range_bucket(300))])
+    x = filled_u32(200, dev)
+    with pytest.raises(gms.GmsError):          # outputs overlap
+        gms.Batch([gms.BatchItem(a, x[:64], 64, 4), gms.BatchItem(a, x[32:96], 64, 4)])
+    with pytest.raises(gms.GmsError):          # output overlaps another item's input
+        gms.Batch([gms.BatchItem(x[:64], o, 64, 4), gms.BatchItem(a, x[32:96], 64, 4)])
+    gms.Batch([gms.BatchItem(a, x[:64], 64, 4), gms.BatchItem(a, x[64:128], 64, 4)])  # shared input ok
+
+
+@pytest.mark.slow
+def test_batch_4096_loguniform_equals_plain_calls(dev):
+    """The batched workload: 4096 key-value multisplits, sizes round(2^U),
+    U ~ U[8, 22], 64 buckets by range, uniform keys.  Each instance equals
+    the plain multisplit of that instance (itself checked against the oracle
+    elsewhere), and a sample of instances equals the oracle directly."""
+    import torch
+    sizes = W.log_uniform_sizes(4096, 8, 22, seed=4096)
+    total = int(sizes.sum())
+    banner(f"batch 4096 instances, {total} keys")
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    kin = torch.randint(-(1 << 31), (1 << 31) - 1, (total,), dtype=torch.int32, device=dev, generator=g)
+    vin = torch.arange(total, dtype=torch.int32, device=dev)
+    kout = torch.empty_like(kin)
+    vout = torch.empty_like(kin)
+    offs = torch.empty(4096 * 64, dtype=torch.int32, device=dev)
+    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    items = [gms.BatchItem(kin[s:s + n], kout[s:s + n], int(n), 64, gms.range_bucket(64),
+                           d_value_in=vin[s:s + n], d_value_out=vout[s:s + n],
+                           d_offsets=offs[64 * i:64 * i + 64])
+             for i, (s, n) in enumerate(zip(starts, sizes))]
+    b = gms.Batch(items)
+    b.autotune(reps=1)
+    b.capture()
+    b.replay()
+    torch.cuda.synchronize()
+    ctx = gms.multisplit_context(64)
+    gms.multisplit_allocate_key_value(int(sizes.max()), ctx)
+    pk = torch.empty(int(sizes.max()), dtype=torch.int32, device=dev)
+    pv = torch.empty_like(pk)
+    po = torch.empty(64, dtype=torch.int32, device=dev)
+    bad = 0
+    for i, (s, n) in enumerate(zip(starts, sizes)):
+        n = int(n)
+        gms.multisplit(kin[s:s + n], pk, n, ctx, gms.range_bucket(64), d_value_in=vin[s:s + n],
+                       d_value_out=pv, d_offsets=po)
+        ok = (torch.equal(pk[:n], kout[s:s + n]) and torch.equal(pv[:n], vout[s:s + n])
+              and torch.equal(po, offs[64 * i:64 * i + 64]))
+        bad += 0 if ok else 1
+    assert bad == 0, f"{bad} instances differ from the plain call"
+    for i in (0, 1, 2047, 4095):
+        s, n = int(starts[i]), int(sizes[i])
+        keys = from_dev(kin[s:s + n])
+        wk, wv, wo = oracle.multisplit(keys, 64, oracle.RANGE, 0, np.arange(s, s + n, dtype=np.uint32))
+        np.testing.assert_array_equal(from_dev(kout[s:s + n]), wk)
+        np.testing.assert_array_equal(from_dev(vout[s:s + n]), wv)
+    gms.multisplit_release_memory(ctx)

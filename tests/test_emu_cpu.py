@@ -103,3 +103,78 @@ def test_emulated_sort(emu, bits):
     order = np.argsort(keys, kind="stable")
     np.testing.assert_array_equal(kout, keys[order])
     np.testing.assert_array_equal(vout, vals[order])
+
+
+# ---------------------------------------------------------------------------
+# Batched multisplit on the emulation: a mixed batch (every bucket class,
+# every identifier kind, key-only and key-value, empty and ragged instances)
+# under every tuning, each instance against the oracle run alone.
+# ---------------------------------------------------------------------------
+BATCH_CASES = [  # (n, B, functor, oracle kind, oracle param, kv)
+    (0, 8, gms.range_bucket(8), oracle.RANGE, 0, True),
+    (1, 2, gms.range_bucket(2), oracle.RANGE, 0, False),
+    (100, 1, gms.range_bucket(1), oracle.RANGE, 0, True),
+    (2047, 5, gms.range_bucket(5), oracle.RANGE, 0, True),
+    (4097, 64, gms.delta_func(1 << 26), oracle.DELTA, 1 << 26, True),
+    (6145, 256, gms.ms_bucket_sort_8bit(8), oracle.DIGIT, 8, False),
+    (3000, 128, gms.identity_bucket(), oracle.IDENTITY, 0, True),
+    (5000, 16, gms.test_bucket(3), oracle.LOWBITS, 3, False),
+    (9000, 256, gms.range_bucket(256), oracle.RANGE, 0, True),
+    (777, 32, gms.ms_bucket_sort_5bit(27), oracle.DIGIT, 27, True),
+]
+
+
+def _emu_batch(L, cases, tuning, seed):
+    keep, items, want = [], [], []
+    for i, (n, B, f, kind, param, kv) in enumerate(cases):
+        keys = W.uniform_keys(n, seed=seed + i)
+        if kind == oracle.IDENTITY:
+            keys = keys % np.uint32(B + 3)        # some ids past B-1: clamped
+        vals = np.arange(n, dtype=np.uint32) + np.uint32(1000 * i)
+        kout = np.full(max(n, 1), 0x5A5A5A5A, np.uint32)
+        vout = np.full(max(n, 1), 0x5A5A5A5A, np.uint32) if kv else None
+        offs = np.full(B, 0xFFFFFFFF, np.uint32)
+        keep += [keys, vals, kout, vout, offs]
+        items.append(gms.gms_batch_item(p(keys) if n else None, p(vals) if kv else None,
+                                        p(kout), p(vout) if kv else None, n, B, f.desc(), p(offs)))
+        wk, wv, wo = oracle.multisplit(keys, B, kind, param, vals if kv else None)
+        want.append((wk, wv, wo))
+    h = ctypes.c_void_p()
+    assert L.gms_batch_create(ctypes.byref(h)) == 0
+    try:
+        assert L.gms_batch_set_tuning(h, gms.gms_batch_tuning(*tuning)) == 0
+        arr = (gms.gms_batch_item * len(items))(*items)
+        assert L.gms_batch_plan(h, arr, len(items), None) == 0
+        assert L.gms_batch_run(h, None) == 0
+    finally:
+        L.gms_batch_destroy(h)
+    for i, (n, B, *_rest) in enumerate(cases):
+        kv = cases[i][5]
+        kout, vout, offs = keep[5 * i + 2], keep[5 * i + 3], keep[5 * i + 4]
+        wk, wv, wo = want[i]
+        np.testing.assert_array_equal(kout[:n], wk, err_msg=f"instance {i} keys")
+        np.testing.assert_array_equal(offs, wo, err_msg=f"instance {i} offsets")
+        if kv:
+            np.testing.assert_array_equal(vout[:n], wv, err_msg=f"instance {i} values")
+
+
+@pytest.mark.parametrize("tuning", [(8, 0), (8, 1), (16, 1), (24, 0)])
+def test_emulated_batch_matches_oracle(emu, tuning):
+    _emu_batch(emu, BATCH_CASES, tuning, seed=sum(tuning))
+
+
+def test_emulated_batch_rejects_overlapping_outputs(emu):
+    a = np.zeros(64, np.uint32)
+    b = np.zeros(64, np.uint32)
+    it1 = gms.gms_batch_item(p(a), None, p(b), None, 32, 4, gms.range_bucket(4).desc(), None)
+    it2 = gms.gms_batch_item(p(a), None, p(b) + 64, None, 32, 4, gms.range_bucket(4).desc(), None)
+    h = ctypes.c_void_p()
+    assert emu.gms_batch_create(ctypes.byref(h)) == 0
+    arr = (gms.gms_batch_item * 2)(it1, it2)
+    assert emu.gms_batch_plan(h, arr, 2, None) == 7           # GMS_ERR_ALIAS
+    it2.d_key_out = p(b) + 128                                  # adjacent: fine
+    arr = (gms.gms_batch_item * 2)(it1, it2)
+    assert emu.gms_batch_plan(h, arr, 2, None) == 0
+    bad = gms.gms_batch_item(p(a), None, p(b), None, 32, 300, gms.range_bucket(4).desc(), None)
+    assert emu.gms_batch_plan(h, (gms.gms_batch_item * 1)(bad), 1, None) == 2
+    emu.gms_batch_destroy(h)
