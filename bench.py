@@ -44,7 +44,91 @@ def parse():
                    help="run one plain copy of n u32 before warm-up (HBM counter calibration)")
     p.add_argument("--no-batched", action="store_true", help="skip the batched-form measurement")
     p.add_argument("--batch-reps", type=int, default=5)
+    p.add_argument("--no-served", action="store_true", help="skip the served-form measurement")
+    p.add_argument("--served-requests", type=int, default=400)
     return p.parse_args()
+
+
+def bench_served(args, torch, gms, dev):
+    """Served form: a mixed request stream (multisplit B in {2,16,64,256} or
+    8-bit radix sort, key-only or key-value, sizes log-uniform [2^10, 2^24],
+    uniform or Zipf 1.1 keys) through the continuous-batching server with a
+    64 GiB scratch budget.  Capacity is measured with all requests arriving
+    at once; then Poisson arrivals at 30/60/90 % of it, reporting throughput
+    and p50/p99 latency.  The same requests as back-to-back plain calls give
+    the comparison throughput."""
+    import numpy as np
+    from gpumultisplit_amd import served as S
+    specs = S.request_specs(args.served_requests, seed=77)
+    total = sum(s["n"] for s in specs)
+    print(f"[bench] served: {len(specs)} requests, {total} keys", file=sys.stderr, flush=True)
+    recs = S.make_requests(gms, torch, specs, dev, seed=78)
+    out = {"workload": f"{len(specs)} requests: multisplit B in {{2,16,64,256}} or 8-bit sort, "
+                       "KO/KV, sizes log-uniform [2^10,2^24], uniform/Zipf 1.1 keys",
+           "requests": len(specs), "total_keys": total,
+           "sorts": sum(1 for s in specs if s["op"] == "sort"), "scratch_budget_gib": 64}
+    with gms.Server(64 << 30, max_inflight_steps=2) as srv:
+        S.run_open_loop(srv, recs[:20], np.zeros(20))           # warm-up
+        lat0, wall0 = S.run_open_loop(srv, recs, np.zeros(len(recs)))
+        cap = len(recs) / wall0
+        out["capacity"] = {"req_per_s": round(cap, 1), "gkeys_per_s": round(total / wall0 / 1e9, 3),
+                           "p50_us": round(S.percentile(lat0, 50), 1),
+                           "p99_us": round(S.percentile(lat0, 99), 1)}
+        rates = []
+        for i, frac in enumerate((0.3, 0.6, 0.9)):
+            arr = S.poisson_arrivals(len(recs), frac * cap, seed=100 + i)
+            lat, wall = S.run_open_loop(srv, recs, arr)
+            rates.append({"offered_req_per_s": round(frac * cap, 1),
+                          "achieved_req_per_s": round(len(recs) / wall, 1),
+                          "gkeys_per_s": round(total / wall / 1e9, 3),
+                          "p50_us": round(S.percentile(lat, 50), 1),
+                          "p99_us": round(S.percentile(lat, 99), 1)})
+        out["poisson"] = rates
+        st = srv.stats()
+        out["steps"] = st["steps"]
+        out["passes_per_step"] = round(st["step_items"] / max(st["steps"], 1), 2)
+        out["peak_scratch_gib"] = round(st["peak_scratch_bytes"] / (1 << 30), 3)
+
+    # the same requests as back-to-back plain calls on one stream
+    nmax = max(s["n"] for s in specs)
+    sort_ctx = gms.ms_sort_context()
+    gms.multisplit_sort_8bit_allocate_pairs(nmax, sort_ctx)
+    ms_ctx = {}
+    for B in (2, 16, 64, 256):
+        ms_ctx[B] = gms.multisplit_context(B)
+        gms.multisplit_allocate_key_value(nmax, ms_ctx[B])
+    L = gms.lib()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def plain_all():
+        for r in recs:
+            s = r["spec"]
+            vin = r["vin"].data_ptr() if s["kv"] else None
+            vout = r["vout"].data_ptr() if s["kv"] else None
+            if s["op"] == "sort":
+                st_ = L.gms_sort(sort_ctx.handle, r["kin"].data_ptr(), r["kout"].data_ptr(), vin,
+                                 vout, s["n"], 8, stream)
+            else:
+                st_ = L.gms_multisplit(ms_ctx[s["B"]].handle, r["kin"].data_ptr(), vin,
+                                       r["kout"].data_ptr(), vout, s["n"],
+                                       gms.range_bucket(s["B"]).desc(), r["offs"].data_ptr(),
+                                       stream)
+            if st_ != 0:
+                raise gms.GmsError(st_, "plain call")
+
+    plain_all()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain_all()
+    torch.cuda.synchronize()
+    wall_plain = time.perf_counter() - t0
+    out["plain_back_to_back"] = {"gkeys_per_s": round(total / wall_plain / 1e9, 3),
+                                 "ms": round(wall_plain * 1e3, 3)}
+    out["capacity_speedup_vs_plain"] = round(wall_plain / wall0, 3)
+    gms.multisplit_sort_release_memory(sort_ctx)
+    for c in ms_ctx.values():
+        gms.multisplit_release_memory(c)
+    return out
 
 
 def bench_batched(args, torch, gms, W, dev):
@@ -270,6 +354,8 @@ def main():
     del kin, vin, kout, vout
     if not args.no_batched:
         out["batched"] = bench_batched(args, torch, gms, W, dev)
+    if not args.no_served:
+        out["served"] = bench_served(args, torch, gms, dev)
     print(json.dumps(out), flush=True)
 
 

@@ -63,6 +63,27 @@ class gms_batch_tuning(ctypes.Structure):
     _fields_ = [("items_per_thread", ctypes.c_uint32), ("persistent", ctypes.c_uint32)]
 
 
+class gms_request(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_uint32), ("d_key_in", ctypes.c_void_p),
+                ("d_value_in", ctypes.c_void_p), ("d_key_out", ctypes.c_void_p),
+                ("d_value_out", ctypes.c_void_p), ("n", ctypes.c_uint32),
+                ("num_buckets", ctypes.c_uint32), ("desc", gms_bucket_desc),
+                ("digit_bits", ctypes.c_uint32), ("d_offsets", ctypes.c_void_p)]
+
+
+class gms_server_config(ctypes.Structure):
+    _fields_ = [("scratch_budget_bytes", ctypes.c_uint64), ("max_inflight_steps", ctypes.c_uint32),
+                ("max_step_keys", ctypes.c_uint32), ("items_per_thread", ctypes.c_uint32)]
+
+
+class gms_server_stats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint64), ("steps", ctypes.c_uint64),
+                ("step_items", ctypes.c_uint64), ("keys_processed", ctypes.c_uint64),
+                ("peak_scratch_bytes", ctypes.c_uint64), ("active", ctypes.c_uint32),
+                ("pending", ctypes.c_uint32)]
+
+
 _lib = None
 
 # (name, restype, argtypes) of every exported symbol, in header order.
@@ -113,6 +134,13 @@ _SIGS = [
     ("gms_batch_get_tuning", _I, [_P, ctypes.POINTER(gms_batch_tuning)]),
     ("gms_batch_stats", _I, [_P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_U32),
                              ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_U32)]),
+    ("gms_server_create", _I, [ctypes.POINTER(gms_server_config), _PP]),
+    ("gms_server_destroy", _I, [_P]),
+    ("gms_server_submit", _I, [_P, ctypes.POINTER(gms_request), ctypes.POINTER(ctypes.c_uint64)]),
+    ("gms_server_poll", _I, [_P, ctypes.c_uint64, ctypes.POINTER(_I),
+                             ctypes.POINTER(ctypes.c_double)]),
+    ("gms_server_wait", _I, [_P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
+    ("gms_server_get_stats", _I, [_P, ctypes.POINTER(gms_server_stats)]),
     ("gms_copy_u32", _I, [_P, _P, ctypes.c_size_t, _P]),
 ]
 EXPORTED_SYMBOLS = [s[0] for s in _SIGS]
@@ -662,6 +690,95 @@ class Batch:
         try:
             if self._h and _lib is not None:
                 _lib.gms_batch_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Request server with continuous batching (no reference counterpart).
+# ---------------------------------------------------------------------------
+OP_MULTISPLIT, OP_SORT = 0, 1
+
+
+def multisplit_request(d_key_in, d_key_out, num_elements: int, num_buckets: int,
+                       bucket_identifier=None, d_value_in=None, d_value_out=None,
+                       d_offsets=None) -> gms_request:
+    f = bucket_identifier if bucket_identifier is not None else range_bucket(num_buckets)
+    for name, t in (("d_key_in", d_key_in), ("d_key_out", d_key_out),
+                    ("d_value_in", d_value_in), ("d_value_out", d_value_out)):
+        _check_len(name, t, num_elements)
+    _check_len("d_offsets", d_offsets, num_buckets)
+    return gms_request(OP_MULTISPLIT, _ptr(d_key_in), _ptr(d_value_in), _ptr(d_key_out),
+                       _ptr(d_value_out), int(num_elements), int(num_buckets), f.desc(), 0,
+                       _ptr(d_offsets))
+
+
+def sort_request(d_key_in, d_key_out, num_elements: int, digit_bits: int = 8, d_value_in=None,
+                 d_value_out=None) -> gms_request:
+    for name, t in (("d_key_in", d_key_in), ("d_key_out", d_key_out),
+                    ("d_value_in", d_value_in), ("d_value_out", d_value_out)):
+        _check_len(name, t, num_elements)
+    return gms_request(OP_SORT, _ptr(d_key_in), _ptr(d_value_in), _ptr(d_key_out),
+                       _ptr(d_value_out), int(num_elements), 0, gms_bucket_desc(0, 0, 0),
+                       int(digit_bits), None)
+
+
+class Server:
+    """Continuous-batching request server on the current device.
+
+    ``submit`` queues a request built by ``multisplit_request`` or
+    ``sort_request`` and returns a ticket; ``wait`` blocks until it is done
+    and returns its latency in microseconds (submit to completion).  The
+    caller keeps the request's buffers alive until then.
+    """
+
+    def __init__(self, scratch_budget_bytes: int = 64 << 30, max_inflight_steps: int = 2,
+                 max_step_keys: int = 0, items_per_thread: int = 24):
+        cfg = gms_server_config(int(scratch_budget_bytes), int(max_inflight_steps),
+                                int(max_step_keys), int(items_per_thread))
+        h = ctypes.c_void_p()
+        _check(lib().gms_server_create(ctypes.byref(cfg), ctypes.byref(h)), "gms_server_create")
+        self._h = h
+
+    def submit(self, req: gms_request) -> int:
+        t = ctypes.c_uint64()
+        _check(lib().gms_server_submit(self._h, ctypes.byref(req), ctypes.byref(t)),
+               "gms_server_submit")
+        return int(t.value)
+
+    def poll(self, ticket: int):
+        d = ctypes.c_int()
+        lat = ctypes.c_double()
+        _check(lib().gms_server_poll(self._h, int(ticket), ctypes.byref(d), ctypes.byref(lat)),
+               "gms_server_poll")
+        return bool(d.value), float(lat.value)
+
+    def wait(self, ticket: int) -> float:
+        lat = ctypes.c_double()
+        _check(lib().gms_server_wait(self._h, int(ticket), ctypes.byref(lat)), "gms_server_wait")
+        return float(lat.value)
+
+    def stats(self):
+        st = gms_server_stats()
+        _check(lib().gms_server_get_stats(self._h, ctypes.byref(st)), "gms_server_get_stats")
+        return {k: int(getattr(st, k)) for k, _ in gms_server_stats._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            _check(lib().gms_server_destroy(self._h), "gms_server_destroy")
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            if self._h and _lib is not None:
+                _lib.gms_server_destroy(self._h)
                 self._h = None
         except Exception:
             pass

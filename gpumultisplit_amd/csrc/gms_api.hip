@@ -203,6 +203,34 @@ gms_status run_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin, 
     return hip_status(e);
 }
 
+int sort_schedule(uint32_t digit_bits, SortPass* p) {
+    int k = 0;
+    switch (digit_bits) {
+        case 8:
+            for (uint32_t s = 0; s < 32; s += 8) p[k++] = {s, 8};
+            break;
+        case 7:
+            p[k++] = {0, 4};
+            for (uint32_t s = 4; s < 32; s += 7) p[k++] = {s, 7};
+            break;
+        case 6:
+            p[k++] = {0, 4};
+            p[k++] = {4, 4};
+            for (uint32_t s = 8; s < 32; s += 6) p[k++] = {s, 6};
+            break;
+        case 5:
+            for (uint32_t s = 0; s < 30; s += 5) p[k++] = {s, 5};
+            p[k++] = {30, 2};
+            break;
+        case 4:
+            for (uint32_t s = 0; s < 32; s += 4) p[k++] = {s, 4};
+            break;
+        default:
+            return 0;
+    }
+    return k;
+}
+
 }  // namespace gms
 
 // ===========================================================================
@@ -393,37 +421,6 @@ gms_status gms_exclusive_scan_u32(gms_multisplit_ctx* ctx, const uint32_t* d_in,
 }
 
 // ---------------------------------------------------------------------- sort
-struct SortPass {
-    uint32_t shift, bits;
-};
-
-static int sort_schedule(uint32_t digit_bits, SortPass* p) {
-    int k = 0;
-    switch (digit_bits) {
-        case 8:
-            for (uint32_t s = 0; s < 32; s += 8) p[k++] = {s, 8};
-            break;
-        case 7:
-            p[k++] = {0, 4};
-            for (uint32_t s = 4; s < 32; s += 7) p[k++] = {s, 7};
-            break;
-        case 6:
-            p[k++] = {0, 4};
-            p[k++] = {4, 4};
-            for (uint32_t s = 8; s < 32; s += 6) p[k++] = {s, 6};
-            break;
-        case 5:
-            for (uint32_t s = 0; s < 30; s += 5) p[k++] = {s, 5};
-            p[k++] = {30, 2};
-            break;
-        case 4:
-            for (uint32_t s = 0; s < 32; s += 4) p[k++] = {s, 4};
-            break;
-        default:
-            return 0;
-    }
-    return k;
-}
 
 gms_status gms_sort_create(gms_sort_ctx** out) {
     if (!out) return GMS_ERR_INVALID_ARG;

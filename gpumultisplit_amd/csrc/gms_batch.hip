@@ -57,9 +57,7 @@ void PinnedVec<T>::release() {
     cap = size = 0;
 }
 
-template struct DevVec<BatchInst>;
-template struct DevVec<BatchWork>;
-template struct DevVec<uint32_t>;
+template struct DevVec<uint8_t>;
 template struct PinnedVec<BatchInst>;
 template struct PinnedVec<BatchWork>;
 
@@ -121,11 +119,17 @@ gms_status BatchEngine::plan(const gms_batch_item* items, uint32_t count, int sl
     PinnedVec<BatchInst>& hi = h_inst_[slot_];
     PinnedVec<BatchWork>& hw = h_work_[slot_];
 
-    // Group order: (class, kv); instances keep their order inside a group.
+    // Group order: (class, kv, alignment); instances keep their order inside
+    // a group.
+    auto aligned = [&](uint32_t i) {
+        return (((uintptr_t)items[i].d_key_in) & 15u) == 0 &&
+               (((uintptr_t)items[i].d_value_in) & 15u) == 0;
+    };
     std::vector<uint32_t> order(count);
     for (uint32_t i = 0; i < count; ++i) order[i] = i;
     auto key = [&](uint32_t i) {
-        return class_of(items[i].num_buckets) * 2 + (items[i].d_value_in ? 1 : 0);
+        return class_of(items[i].num_buckets) * 4 + (items[i].d_value_in ? 2 : 0) +
+               (aligned(i) ? 0 : 1);
     };
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t a, uint32_t b) { return key(a) < key(b); });
@@ -145,8 +149,10 @@ gms_status BatchEngine::plan(const gms_batch_item* items, uint32_t count, int sl
         const gms_batch_item& it = items[order[j]];
         const int lb = class_of(it.num_buckets);
         const bool kv = it.d_value_in != nullptr;
-        if (groups.empty() || groups.back().log_b != lb || groups.back().kv != kv)
-            groups.push_back(BatchGroup{lb, kv, w, w});
+        const bool al = aligned(order[j]);
+        if (groups.empty() || groups.back().log_b != lb || groups.back().kv != kv ||
+            groups.back().vec != al)
+            groups.push_back(BatchGroup{lb, kv, al, w, w});
         const uint32_t nt = it.n ? (uint32_t)(((uint64_t)it.n + tile - 1) / tile) : 1u;
         BatchInst& I = hi.p[j];
         I.kin = it.d_key_in;
@@ -159,9 +165,6 @@ gms_status BatchEngine::plan(const gms_batch_item* items, uint32_t count, int sl
         I.mbase = (uint32_t)mbase;
         I.obase = (uint32_t)obase;
         I.nb = it.num_buckets;
-        const bool al = (((uintptr_t)it.d_key_in) & 15u) == 0 &&
-                        (((uintptr_t)it.d_value_in) & 15u) == 0;
-        I.vec = al ? 1u : 0u;
         I.f = AnyF::make(it.desc.kind, it.desc.param, it.desc.width, it.num_buckets);
         for (uint32_t t = 0; t < nt; ++t) hw.p[w++] = BatchWork{j, t};
         groups.back().w1 = w;
@@ -178,7 +181,32 @@ gms_status BatchEngine::plan(const gms_batch_item* items, uint32_t count, int sl
     return GMS_SUCCESS;
 }
 
-gms_status BatchEngine::upload(hipStream_t s) {
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t BatchEngine::device_bytes() const {
+    const size_t m = std::max<uint64_t>(m_total, 4);
+    return align256(std::max<size_t>(num_instances, 1) * sizeof(BatchInst)) +
+           align256(std::max<size_t>(num_work, 1) * sizeof(BatchWork)) + align256(m * 4) +
+           align256(scan_partials_needed(m) * 4) + align256(16 * std::max<size_t>(groups.size(), 1) * 4);
+}
+
+BatchDevice BatchEngine::carve(void* base) const {
+    const size_t m = std::max<uint64_t>(m_total, 4);
+    char* p = (char*)base;
+    BatchDevice d;
+    d.inst = (BatchInst*)p;
+    p += align256(std::max<size_t>(num_instances, 1) * sizeof(BatchInst));
+    d.work = (BatchWork*)p;
+    p += align256(std::max<size_t>(num_work, 1) * sizeof(BatchWork));
+    d.mat = (uint32_t*)p;
+    p += align256(m * 4);
+    d.part = (uint32_t*)p;
+    p += align256(scan_partials_needed(m) * 4);
+    d.queue = (uint32_t*)p;
+    return d;
+}
+
+gms_status BatchEngine::upload(const BatchDevice& d, hipStream_t s) {
     if (num_cus_ == 0) {
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess) return GMS_ERR_NO_DEVICE;
@@ -186,18 +214,12 @@ gms_status BatchEngine::upload(hipStream_t s) {
             return GMS_ERR_HIP;
         num_cus_ = cus > 0 ? cus : 1;
     }
-    gms_status st = d_inst_.reserve(std::max<uint32_t>(num_instances, 1), s);
-    if (st == GMS_SUCCESS) st = d_work_.reserve(std::max<uint32_t>(num_work, 1), s);
-    if (st == GMS_SUCCESS) st = d_mat_.reserve(std::max<uint64_t>(m_total, 4), s);
-    if (st == GMS_SUCCESS) st = d_part_.reserve(scan_partials_needed(std::max<uint64_t>(m_total, 4)), s);
-    if (st == GMS_SUCCESS) st = d_queue_.reserve(2 * std::max<size_t>(groups.size(), 1), s);
-    if (st != GMS_SUCCESS) return st;
     if (num_instances &&
-        hipMemcpyAsync(d_inst_.p, h_inst_[slot_].p, (size_t)num_instances * sizeof(BatchInst),
+        hipMemcpyAsync(d.inst, h_inst_[slot_].p, (size_t)num_instances * sizeof(BatchInst),
                        hipMemcpyHostToDevice, s) != hipSuccess)
         return GMS_ERR_HIP;
     if (num_work &&
-        hipMemcpyAsync(d_work_.p, h_work_[slot_].p, (size_t)num_work * sizeof(BatchWork),
+        hipMemcpyAsync(d.work, h_work_[slot_].p, (size_t)num_work * sizeof(BatchWork),
                        hipMemcpyHostToDevice, s) != hipSuccess)
         return GMS_ERR_HIP;
     return GMS_SUCCESS;
@@ -212,23 +234,25 @@ uint32_t BatchEngine::grid_for(const BatchGroup& g, bool post) const {
         case 16: occ = batch_occupancy_k16(g.log_b, g.kv, post); break;
         default: occ = batch_occupancy_k24(g.log_b, g.kv, post); break;
     }
-    const uint64_t resident = (uint64_t)occ * (uint64_t)num_cus_;
+    // At least 8 workgroups (one per XCD) whenever there are 8 items, so that
+    // every XCD's range of items has a workgroup to drain it.
+    const uint64_t resident = std::max<uint64_t>((uint64_t)occ * (uint64_t)num_cus_, 8);
     return (uint32_t)std::min<uint64_t>(nw, resident);
 }
 
 uint32_t BatchEngine::launches() const {
     if (num_work == 0) return 0;
-    return 1u + 2u * (uint32_t)groups.size() + (m_total ? 3u : 0u);
+    return (tuning.persistent ? 1u : 0u) + 2u * (uint32_t)groups.size() + (m_total ? 3u : 0u);
 }
 
-gms_status BatchEngine::launch(hipStream_t s) {
+gms_status BatchEngine::launch(const BatchDevice& d, hipStream_t s) {
     if (num_work == 0) return GMS_SUCCESS;
     const uint32_t G = (uint32_t)groups.size();
-    hipError_t e = launch_fill(d_queue_.p, 0u, 2 * G, s);
+    hipError_t e = tuning.persistent ? launch_fill(d.queue, 0u, 16 * G, s) : hipSuccess;
     auto one = [&](const BatchGroup& g, uint32_t gi, bool post) {
-        BatchLaunch c{d_inst_.p, d_work_.p + g.w0, g.w1 - g.w0, grid_for(g, post), d_mat_.p,
-                      (uint32_t)m_total, (uint32_t)keys_total, d_queue_.p + 2 * gi + (post ? 1 : 0),
-                      g.log_b, g.kv, s};
+        BatchLaunch c{d.inst, d.work + g.w0, g.w1 - g.w0, grid_for(g, post), d.mat,
+                      (uint32_t)m_total, (uint32_t)keys_total, d.queue + 16 * gi + (post ? 8 : 0),
+                      g.log_b, g.kv, g.vec, tuning.persistent != 0, s};
         switch (tuning.items_per_thread) {
             case 8: return post ? launch_batch_postscan_k8(c) : launch_batch_prescan_k8(c);
             case 16: return post ? launch_batch_postscan_k16(c) : launch_batch_prescan_k16(c);
@@ -237,7 +261,7 @@ gms_status BatchEngine::launch(hipStream_t s) {
     };
     for (uint32_t gi = 0; gi < G && e == hipSuccess; ++gi) e = one(groups[gi], gi, false);
     if (e == hipSuccess)
-        e = launch_exclusive_scan(d_mat_.p, d_mat_.p, (uint32_t)m_total, d_part_.p, s);
+        e = launch_exclusive_scan(d.mat, d.mat, (uint32_t)m_total, d.part, s);
     for (uint32_t gi = 0; gi < G && e == hipSuccess; ++gi) e = one(groups[gi], gi, true);
     return hip_status(e);
 }
@@ -247,11 +271,6 @@ void BatchEngine::release() {
         h_inst_[i].release();
         h_work_[i].release();
     }
-    d_inst_.release();
-    d_work_.release();
-    d_mat_.release();
-    d_part_.release();
-    d_queue_.release();
     num_work = num_instances = 0;
     m_total = keys_total = 0;
     groups.clear();
@@ -263,6 +282,8 @@ using namespace gms;
 
 struct gms_batch {
     BatchEngine eng;
+    DevVec<uint8_t> mem;
+    BatchDevice dev;
     std::vector<gms_batch_item> items;
     bool planned = false;
     hipGraph_t graph = nullptr;
@@ -279,7 +300,11 @@ struct gms_batch {
         drop_graph();
         planned = false;
         gms_status st = eng.plan(items.data(), (uint32_t)items.size(), 0);
-        if (st == GMS_SUCCESS) st = eng.upload(s);
+        if (st == GMS_SUCCESS) st = mem.reserve(eng.device_bytes(), s);
+        if (st == GMS_SUCCESS) {
+            dev = eng.carve(mem.p);
+            st = eng.upload(dev, s);
+        }
         if (st == GMS_SUCCESS) st = hip_status(hipStreamSynchronize(s));
         planned = st == GMS_SUCCESS;
         return st;
@@ -299,6 +324,7 @@ gms_status gms_batch_destroy(gms_batch* b) {
     b->drop_graph();
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     b->eng.release();
+    b->mem.release();
     delete b;
     return GMS_SUCCESS;
 }
@@ -319,7 +345,7 @@ gms_status gms_batch_plan(gms_batch* b, const gms_batch_item* items, uint32_t co
 gms_status gms_batch_run(gms_batch* b, void* stream) {
     if (!b) return GMS_ERR_INVALID_ARG;
     if (!b->planned) return GMS_ERR_NOT_ALLOCATED;
-    return b->eng.launch((hipStream_t)stream);
+    return b->eng.launch(b->dev, (hipStream_t)stream);
 }
 
 gms_status gms_batch_capture(gms_batch* b) {
@@ -332,7 +358,7 @@ gms_status gms_batch_capture(gms_batch* b) {
     }
     if (hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return GMS_ERR_HIP;
-    gms_status st = b->eng.launch(b->cap_stream);
+    gms_status st = b->eng.launch(b->dev, b->cap_stream);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
     if (st != GMS_SUCCESS || e != hipSuccess) {
@@ -390,9 +416,9 @@ gms_status gms_batch_autotune(gms_batch* b, uint32_t reps, void* stream, gms_bat
             b->eng.tuning.items_per_thread = K;
             b->eng.tuning.persistent = pers;
             st = b->replan(s);
-            if (st == GMS_SUCCESS) st = b->eng.launch(s);  // warm-up
+            if (st == GMS_SUCCESS) st = b->eng.launch(b->dev, s);  // warm-up
             if (st == GMS_SUCCESS) st = hip_status(hipEventRecord(e0, s));
-            for (uint32_t r = 0; r < reps && st == GMS_SUCCESS; ++r) st = b->eng.launch(s);
+            for (uint32_t r = 0; r < reps && st == GMS_SUCCESS; ++r) st = b->eng.launch(b->dev, s);
             if (st == GMS_SUCCESS) st = hip_status(hipEventRecord(e1, s));
             if (st == GMS_SUCCESS) st = hip_status(hipEventSynchronize(e1));
             float ms = 0.f;

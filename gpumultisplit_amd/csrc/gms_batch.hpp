@@ -37,6 +37,7 @@ struct PinnedVec {
 struct BatchGroup {
     int log_b;
     bool kv;
+    bool vec;         // every instance 16-byte aligned
     uint32_t w0, w1;  // work items [w0, w1)
 };
 
@@ -45,16 +46,30 @@ struct BatchTuning {
     uint32_t persistent = 0;         // 0: one workgroup per tile; 1: resident grid + work queue
 };
 
+// Device tables of one planned batch, carved from one allocation.
+struct BatchDevice {
+    BatchInst* inst = nullptr;
+    BatchWork* work = nullptr;
+    uint32_t* mat = nullptr;    // concatenated histogram matrix
+    uint32_t* part = nullptr;   // scan partials
+    uint32_t* queue = nullptr;  // 8 per-XCD work-queue counters per launch
+};
+
 class BatchEngine {
   public:
     BatchTuning tuning;
     // Plan `count` instances (validated).  Host tables go to staging slot
-    // `slot` (0 or 1): the caller makes sure no copy from that slot is still
-    // pending.  Tables are copied to the device on `stream` by upload().
+    // `slot` (0 or 1) of pinned memory: the caller makes sure no copy from
+    // that slot is still pending.
     gms_status plan(const gms_batch_item* items, uint32_t count, int slot);
-    gms_status upload(hipStream_t stream);
+    // Device bytes the current plan needs, and its tables inside `base`
+    // (256-byte aligned, device_bytes() long).
+    size_t device_bytes() const;
+    BatchDevice carve(void* base) const;
+    // Copies the staged tables to the device on `stream` (asynchronous).
+    gms_status upload(const BatchDevice& d, hipStream_t stream);
     // Queue reset, prescans, one scan, postscans -- all kernels, no memsets.
-    gms_status launch(hipStream_t stream);
+    gms_status launch(const BatchDevice& d, hipStream_t stream);
     void release();
 
     uint64_t keys_total = 0;
@@ -68,11 +83,6 @@ class BatchEngine {
     int slot_ = 0;
     PinnedVec<BatchInst> h_inst_[2];
     PinnedVec<BatchWork> h_work_[2];
-    DevVec<BatchInst> d_inst_;
-    DevVec<BatchWork> d_work_;
-    DevVec<uint32_t> d_mat_;
-    DevVec<uint32_t> d_part_;
-    DevVec<uint32_t> d_queue_;
     int num_cus_ = 0;
     uint32_t grid_for(const BatchGroup& g, bool post) const;
 };

@@ -2,11 +2,9 @@
 // prescan launch, one scan and one postscan launch per (bucket class,
 // key-value) group, whatever the number of instances.
 //
-// Work is a list of (instance, tile) items.  Each kernel is persistent: a
-// workgroup starts on item blockIdx.x and then takes the next item from a
-// device counter (gridDim.x + atomicAdd), until the items run out -- every
-// workgroup reaches that exit.  With as many workgroups as items this is one
-// tile per workgroup; with a resident grid it is a work queue.
+// Work is a list of (instance, tile) items, processed either one item per
+// workgroup or by a resident grid pulling items from per-XCD work queues
+// (see xcd_range below).
 //
 // The histogram matrices of all instances are concatenated (instance i's
 // 2^LOG_B x tiles_i block at mbase_i) and scanned as ONE array.  The scan of
@@ -32,7 +30,6 @@ struct BatchInst {
     uint32_t mbase;  // first word of this instance's matrix block
     uint32_t obase;  // keys of all earlier instances of the batch
     uint32_t nb;     // bucket count B
-    uint32_t vec;    // key (and value) pointers 16-byte aligned
     AnyF f;
 };
 
@@ -41,42 +38,120 @@ struct BatchWork {
     uint32_t tile;
 };
 
-// Next work item of this workgroup (uniform); the barrier also separates the
-// previous tile's last LDS reads from the next tile's writes.
-__device__ __forceinline__ uint32_t batch_next(uint32_t* queue, uint32_t* s_next) {
-    if (threadIdx.x == 0) *s_next = gridDim.x + atomicAdd(queue, 1u);
+// Work order.  Workgroups go to the 8 XCDs round-robin (workgroup i on XCD
+// i mod 8).  Every XCD owns one contiguous range of work items (the split of
+// xcd_tile), so neighbouring tiles of an instance -- which share cache lines
+// of the histogram matrix -- meet in the same L2.
+//  * one workgroup per item: item = xcd_tile(blockIdx.x), no atomics;
+//  * resident grid: the workgroups of XCD x take the items of its range in
+//    order, first blockIdx.x / 8, then from a counter of that XCD
+//    (wg_x + atomicAdd), until the range is exhausted -- every workgroup
+//    reaches that exit.
+struct XcdRange {
+    uint32_t begin, count, wgs;  // items [begin, begin+count), workgroups on this XCD
+};
+
+__device__ __forceinline__ XcdRange xcd_range(uint32_t num_work) {
+    constexpr uint32_t NX = 8;
+    const uint32_t x = blockIdx.x % NX;
+    const uint32_t q = num_work / NX, r = num_work % NX;
+    const uint32_t g = gridDim.x;
+    return XcdRange{x * q + (x < r ? x : r), q + (x < r ? 1u : 0u), g / NX + (x < g % NX ? 1u : 0u)};
+}
+
+// Next local item of this workgroup's XCD range (uniform); the barrier also
+// separates the previous tile's last LDS reads from the next tile's writes.
+__device__ __forceinline__ uint32_t xcd_next(uint32_t* queue, uint32_t wgs, uint32_t* s_next) {
+    if (threadIdx.x == 0) *s_next = wgs + atomicAdd(queue + blockIdx.x % 8u, 1u);
     __syncthreads();
     return (uint32_t)GMS_READFIRSTLANE((int)*s_next);
 }
 
-template <int LOG_B, int K>
+template <int LOG_B, int K, bool VEC>
+__device__ __forceinline__ void batch_prescan_item(const BatchInst* __restrict__ inst,
+                                                   const BatchWork* __restrict__ work, uint32_t w,
+                                                   uint32_t* __restrict__ H, uint32_t* s_h) {
+    constexpr uint32_t TILE = kThreads * K;
+    const BatchWork it = work[w];
+    const BatchInst& I = inst[it.inst];
+    const uint32_t nvalid = tile_valid(I.n, it.tile, TILE);
+    const size_t base = (size_t)it.tile * TILE;
+    uint32_t* Hi = H + I.mbase;
+    uint32_t x[K];
+    if (nvalid == TILE) {
+        load_striped4_full<K, VEC>(I.kin, base, x);
+        prescan_tile<LOG_B, K, true>(x, TILE, it.tile, I.num_tiles, Hi, I.f, s_h);
+    } else {
+        load_striped4_guarded<K>(I.kin, base, nvalid, x);
+        prescan_tile<LOG_B, K, false>(x, nvalid, it.tile, I.num_tiles, Hi, I.f, s_h);
+    }
+}
+
+template <int LOG_B, int K, bool VEC, bool PERSIST>
 __global__ __launch_bounds__(kThreads) void ms_batch_prescan_kernel(
     const BatchInst* __restrict__ inst, const BatchWork* __restrict__ work, uint32_t num_work,
     uint32_t* __restrict__ H, uint32_t* __restrict__ queue) {
     constexpr int B = 1 << LOG_B;
-    constexpr uint32_t TILE = kThreads * K;
     __shared__ __attribute__((aligned(16))) uint32_t s_h[kWaves * B];
-    __shared__ uint32_t s_next;
-
-    for (uint32_t w = blockIdx.x; w < num_work; w = batch_next(queue, &s_next)) {
-        const BatchWork it = work[w];
-        const BatchInst& I = inst[it.inst];
-        const uint32_t nvalid = tile_valid(I.n, it.tile, TILE);
-        const size_t base = (size_t)it.tile * TILE;
-        uint32_t* Hi = H + I.mbase;
-        uint32_t x[K];
-        if (nvalid == TILE) {
-            if (I.vec) load_striped4_full<K, true>(I.kin, base, x);
-            else load_striped4_full<K, false>(I.kin, base, x);
-            prescan_tile<LOG_B, K, true>(x, TILE, it.tile, I.num_tiles, Hi, I.f, s_h);
-        } else {
-            load_striped4_guarded<K>(I.kin, base, nvalid, x);
-            prescan_tile<LOG_B, K, false>(x, nvalid, it.tile, I.num_tiles, Hi, I.f, s_h);
-        }
+    if constexpr (!PERSIST) {
+        batch_prescan_item<LOG_B, K, VEC>(inst, work, xcd_tile(blockIdx.x, num_work), H, s_h);
+    } else {
+        __shared__ uint32_t s_next;
+        const XcdRange R = xcd_range(num_work);
+        for (uint32_t i = blockIdx.x / 8u; i < R.count; i = xcd_next(queue, R.wgs, &s_next))
+            batch_prescan_item<LOG_B, K, VEC>(inst, work, R.begin + i, H, s_h);
     }
 }
 
-template <int LOG_B, int K, int LEVEL_BITS, bool KV>
+template <int LOG_B, int K, int LEVEL_BITS, bool KV, bool VEC>
+__device__ __forceinline__ void batch_postscan_item(
+    const BatchInst* __restrict__ inst, const BatchWork* __restrict__ work, uint32_t w,
+    const uint32_t* __restrict__ G, uint32_t m_total, uint32_t keys_total, uint32_t* s_keys,
+    uint32_t* s_vals, uint32_t* s_cnt, uint32_t* s_delta, uint32_t* s_bs, uint32_t* s_ws) {
+    constexpr int B = 1 << LOG_B;
+    constexpr uint32_t TILE = kThreads * K;
+    const uint32_t tid = threadIdx.x;
+    const BatchWork it = work[w];
+    const BatchInst& I = inst[it.inst];
+    const uint32_t t = it.tile;
+    const uint32_t nvalid = tile_valid(I.n, t, TILE);
+    const size_t base = (size_t)t * TILE;
+
+    // G[b][t] of this instance and the next word in scan order; the word
+    // after the whole concatenation is the batch's key total.
+    const uint32_t bt = tid < (uint32_t)B ? tid : (uint32_t)B - 1u;
+    const uint32_t idx = I.mbase + bt * I.num_tiles + t;
+    const uint32_t g0 = G[idx];
+    const uint32_t g1 = idx + 1u < m_total ? G[idx + 1u] : keys_total;
+    const uint32_t cntb = g1 - g0;
+    const uint32_t g0l = g0 - I.obase;  // start inside this instance's output
+    if (t == 0 && I.offs && tid < I.nb) I.offs[tid] = g0l;
+
+    uint32_t key[K], val[K];
+    if (nvalid == TILE) {
+        load_blocked_full<K, VEC>(I.kin, base, key);
+        if constexpr (KV) load_blocked_full<K, VEC>(I.vin, base, val);
+        else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) val[k] = 0;
+        }
+        postscan_rank<LOG_B, K, LEVEL_BITS, KV, true>(key, val, g0l, cntb, TILE, I.f, s_keys,
+                                                      s_vals, s_cnt, s_delta, s_bs, s_ws);
+        postscan_write<K, KV, true>(TILE, I.n, I.kout, I.vout, I.f, s_keys, s_vals, s_delta);
+    } else {
+        load_blocked_guarded<K>(I.kin, base, nvalid, key);
+        if constexpr (KV) load_blocked_guarded<K>(I.vin, base, nvalid, val);
+        else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) val[k] = 0;
+        }
+        postscan_rank<LOG_B, K, LEVEL_BITS, KV, false>(key, val, g0l, cntb, nvalid, I.f, s_keys,
+                                                       s_vals, s_cnt, s_delta, s_bs, s_ws);
+        postscan_write<K, KV, false>(nvalid, I.n, I.kout, I.vout, I.f, s_keys, s_vals, s_delta);
+    }
+}
+
+template <int LOG_B, int K, int LEVEL_BITS, bool KV, bool VEC, bool PERSIST>
 __global__ __launch_bounds__(kThreads) void ms_batch_postscan_kernel(
     const BatchInst* __restrict__ inst, const BatchWork* __restrict__ work, uint32_t num_work,
     const uint32_t* __restrict__ G, uint32_t m_total, uint32_t keys_total,
@@ -93,55 +168,19 @@ __global__ __launch_bounds__(kThreads) void ms_batch_postscan_kernel(
     __shared__ uint32_t s_delta[B];
     __shared__ uint32_t s_bs[kWaves];
     __shared__ uint32_t s_ws[kWaves];
-    __shared__ uint32_t s_next;
 
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t w = blockIdx.x; w < num_work; w = batch_next(queue, &s_next)) {
-        const BatchWork it = work[w];
-        const BatchInst& I = inst[it.inst];
-        const uint32_t t = it.tile;
-        const uint32_t nvalid = tile_valid(I.n, t, TILE);
-        const size_t base = (size_t)t * TILE;
-
-        // G[b][t] of this instance and the next word in scan order; the word
-        // after the whole concatenation is the batch's key total.
-        const uint32_t bt = tid < (uint32_t)B ? tid : (uint32_t)B - 1u;
-        const uint32_t idx = I.mbase + bt * I.num_tiles + t;
-        const uint32_t g0 = G[idx];
-        const uint32_t g1 = idx + 1u < m_total ? G[idx + 1u] : keys_total;
-        const uint32_t cntb = g1 - g0;
-        const uint32_t g0l = g0 - I.obase;  // start inside this instance's output
-        if (t == 0 && I.offs && tid < I.nb) I.offs[tid] = g0l;
-
-        uint32_t key[K], val[K];
-        if (nvalid == TILE) {
-            if (I.vec) {
-                load_blocked_full<K, true>(I.kin, base, key);
-                if constexpr (KV) load_blocked_full<K, true>(I.vin, base, val);
-            } else {
-                load_blocked_full<K, false>(I.kin, base, key);
-                if constexpr (KV) load_blocked_full<K, false>(I.vin, base, val);
-            }
-            if constexpr (!KV) {
-#pragma unroll
-                for (int k = 0; k < K; ++k) val[k] = 0;
-            }
-            postscan_rank<LOG_B, K, LEVEL_BITS, KV, true>(key, val, g0l, cntb, TILE, I.f, s_keys,
-                                                          s_vals, s_cnt, s_delta, s_bs, s_ws);
-            postscan_write<K, KV, true>(TILE, I.n, I.kout, I.vout, I.f, s_keys, s_vals, s_delta);
-        } else {
-            load_blocked_guarded<K>(I.kin, base, nvalid, key);
-            if constexpr (KV) load_blocked_guarded<K>(I.vin, base, nvalid, val);
-            else {
-#pragma unroll
-                for (int k = 0; k < K; ++k) val[k] = 0;
-            }
-            postscan_rank<LOG_B, K, LEVEL_BITS, KV, false>(key, val, g0l, cntb, nvalid, I.f,
-                                                           s_keys, s_vals, s_cnt, s_delta, s_bs,
-                                                           s_ws);
-            postscan_write<K, KV, false>(nvalid, I.n, I.kout, I.vout, I.f, s_keys, s_vals,
-                                         s_delta);
-        }
+    if constexpr (!PERSIST) {
+        batch_postscan_item<LOG_B, K, LEVEL_BITS, KV, VEC>(
+            inst, work, xcd_tile(blockIdx.x, num_work), G, m_total, keys_total, s_keys, s_vals,
+            s_cnt, s_delta, s_bs, s_ws);
+    } else {
+        __shared__ uint32_t s_next;
+        const XcdRange R = xcd_range(num_work);
+        for (uint32_t i = blockIdx.x / 8u; i < R.count; i = xcd_next(queue, R.wgs, &s_next))
+            batch_postscan_item<LOG_B, K, LEVEL_BITS, KV, VEC>(inst, work, R.begin + i, G,
+                                                                m_total, keys_total, s_keys,
+                                                                s_vals, s_cnt, s_delta, s_bs,
+                                                                s_ws);
     }
 }
 
@@ -150,13 +189,15 @@ struct BatchLaunch {
     const BatchInst* inst;
     const BatchWork* work;  // this group's items
     uint32_t num_work;
-    uint32_t grid;          // workgroups (<= num_work)
+    uint32_t grid;          // workgroups (num_work, or the resident grid)
     uint32_t* H;            // whole concatenated matrix
     uint32_t m_total;
     uint32_t keys_total;
-    uint32_t* queue;        // this launch's counter (zero on entry)
+    uint32_t* queue;        // this launch's 8 per-XCD counters (zero on entry)
     int log_b;
     bool kv;
+    bool vec;               // every instance of the group 16-byte aligned
+    bool persistent;
     hipStream_t stream;
 };
 
@@ -166,26 +207,38 @@ hipError_t launch_batch_prescan_k24(const BatchLaunch& c);
 hipError_t launch_batch_postscan_k8(const BatchLaunch& c);
 hipError_t launch_batch_postscan_k16(const BatchLaunch& c);
 hipError_t launch_batch_postscan_k24(const BatchLaunch& c);
-// Resident workgroups per CU of the batch kernels (for the work-queue grid).
+// Resident workgroups per CU of the persistent batch kernels.
 int batch_occupancy_k8(int log_b, bool kv, bool postscan);
 int batch_occupancy_k16(int log_b, bool kv, bool postscan);
 int batch_occupancy_k24(int log_b, bool kv, bool postscan);
 
 template <int K>
 struct BatchKernels {
+    template <int LOG_B, bool VEC, bool P>
+    static hipError_t prescan_v(const BatchLaunch& c) {
+        hipLaunchKernelGGL((ms_batch_prescan_kernel<LOG_B, K, VEC, P>), dim3(c.grid),
+                           dim3(kThreads), 0, c.stream, c.inst, c.work, c.num_work, c.H, c.queue);
+        return hipGetLastError();
+    }
     template <int LOG_B>
     static hipError_t prescan(const BatchLaunch& c) {
-        hipLaunchKernelGGL((ms_batch_prescan_kernel<LOG_B, K>), dim3(c.grid), dim3(kThreads), 0,
-                           c.stream, c.inst, c.work, c.num_work, c.H, c.queue);
+        if (c.persistent)
+            return c.vec ? prescan_v<LOG_B, true, true>(c) : prescan_v<LOG_B, false, true>(c);
+        return c.vec ? prescan_v<LOG_B, true, false>(c) : prescan_v<LOG_B, false, false>(c);
+    }
+    template <int LOG_B, bool KV, bool VEC, bool P>
+    static hipError_t postscan_v(const BatchLaunch& c) {
+        constexpr int LB = tune_level_bits(LOG_B);
+        hipLaunchKernelGGL((ms_batch_postscan_kernel<LOG_B, K, LB, KV, VEC, P>), dim3(c.grid),
+                           dim3(kThreads), 0, c.stream, c.inst, c.work, c.num_work,
+                           (const uint32_t*)c.H, c.m_total, c.keys_total, c.queue);
         return hipGetLastError();
     }
     template <int LOG_B, bool KV>
     static hipError_t postscan(const BatchLaunch& c) {
-        constexpr int LB = tune_level_bits(LOG_B);
-        hipLaunchKernelGGL((ms_batch_postscan_kernel<LOG_B, K, LB, KV>), dim3(c.grid),
-                           dim3(kThreads), 0, c.stream, c.inst, c.work, c.num_work,
-                           (const uint32_t*)c.H, c.m_total, c.keys_total, c.queue);
-        return hipGetLastError();
+        if (c.persistent)
+            return c.vec ? postscan_v<LOG_B, KV, true, true>(c) : postscan_v<LOG_B, KV, false, true>(c);
+        return c.vec ? postscan_v<LOG_B, KV, true, false>(c) : postscan_v<LOG_B, KV, false, false>(c);
     }
     template <int LOG_B, bool KV>
     static int occupancy(bool post) {
@@ -194,10 +247,10 @@ struct BatchKernels {
         if (post) {
             constexpr int LB = tune_level_bits(LOG_B);
             e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &occ, ms_batch_postscan_kernel<LOG_B, K, LB, KV>, kThreads, 0);
+                &occ, ms_batch_postscan_kernel<LOG_B, K, LB, KV, true, true>, kThreads, 0);
         } else {
             e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &occ, ms_batch_prescan_kernel<LOG_B, K>, kThreads, 0);
+                &occ, ms_batch_prescan_kernel<LOG_B, K, true, true>, kThreads, 0);
         }
         return (e == hipSuccess && occ > 0) ? occ : 1;
     }

@@ -88,6 +88,14 @@ gms_status run_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin, 
 gms_status run_prescan_only(Scratch& s, const uint32_t* kin, uint32_t n, uint32_t num_buckets,
                             gms_bucket_desc d, bool kv, hipStream_t stream, uint32_t* num_tiles);
 
+// LSD digit schedule of the radix sort for digit_bits in {4..8}: the
+// reference's passes (multisplit_sort.cuh:320-559), least significant first.
+// Returns the pass count (0 for an unsupported width); p holds up to 8.
+struct SortPass {
+    uint32_t shift, bits;
+};
+int sort_schedule(uint32_t digit_bits, SortPass* p);
+
 gms_status hip_status(hipError_t e);
 bool ranges_overlap(const void* a, size_t an, const void* b, size_t bn);
 

@@ -280,6 +280,72 @@ gms_status gms_batch_stats(const gms_batch *b, uint64_t *keys, uint32_t *work_it
                            uint64_t *matrix_words, uint32_t *launches);
 
 /* ------------------------------------------------------------------------ */
+/* Request server with continuous batching (no counterpart in the reference) */
+/* ------------------------------------------------------------------------ */
+
+/* A request: one multisplit (as gms_multisplit) or one radix sort (as
+ * gms_sort).  Buffers are device memory the caller keeps alive and does not
+ * touch until the request has completed. */
+typedef enum gms_request_op {
+    GMS_OP_MULTISPLIT = 0,
+    GMS_OP_SORT = 1
+} gms_request_op;
+
+typedef struct gms_request {
+    uint32_t op;                  /* gms_request_op */
+    const uint32_t *d_key_in;
+    const uint32_t *d_value_in;   /* NULL = key-only */
+    uint32_t *d_key_out;
+    uint32_t *d_value_out;
+    uint32_t n;
+    uint32_t num_buckets;         /* multisplit: 1..256 */
+    gms_bucket_desc desc;         /* multisplit: bucket identifier */
+    uint32_t digit_bits;          /* sort: 4..8 */
+    uint32_t *d_offsets;          /* multisplit: B bucket starts (device) or NULL */
+} gms_request;
+
+typedef struct gms_server_config {
+    uint64_t scratch_budget_bytes;  /* device scratch arena; admission is bounded by it */
+    uint32_t max_inflight_steps;    /* steps queued on the GPU at once (1 or 2) */
+    uint32_t max_step_keys;         /* keys per step (0 = no limit below 2^31) */
+    uint32_t items_per_thread;      /* batched-kernel tile shape: 8, 16 or 24 */
+} gms_server_config;
+
+typedef struct gms_server_stats {
+    uint64_t submitted, completed, rejected;
+    uint64_t steps;                 /* batched launches sequences issued */
+    uint64_t step_items;            /* passes summed over steps */
+    uint64_t keys_processed;        /* keys summed over all passes */
+    uint64_t peak_scratch_bytes;    /* high-water mark of the arena */
+    uint32_t active, pending;       /* requests admitted / waiting now */
+} gms_server_stats;
+
+typedef struct gms_server gms_server;
+
+/* Allocates the scratch arena on the current device and starts the
+ * scheduler thread.  Every step gathers the next pass of every admitted
+ * request (a multisplit has one pass, a sort one per digit) into one batched
+ * multisplit; requests that arrive while steps run join the next step. */
+gms_status gms_server_create(const gms_server_config *cfg, gms_server **out);
+
+/* Stops accepting requests, finishes every submitted one, frees everything. */
+gms_status gms_server_destroy(gms_server *s);
+
+/* Queues a request (thread-safe, does not block on the GPU).  Requests wait
+ * in arrival order until their scratch fits in the budget; a request that
+ * could never fit is rejected with GMS_ERR_CAPACITY. */
+gms_status gms_server_submit(gms_server *s, const gms_request *req, uint64_t *ticket);
+
+/* done = 1 once the request's results are in its output buffers; latency_us
+ * = submit to completion as seen by the scheduler. */
+gms_status gms_server_poll(gms_server *s, uint64_t ticket, int *done, double *latency_us);
+
+/* Blocks until the request completes, then forgets the ticket. */
+gms_status gms_server_wait(gms_server *s, uint64_t ticket, double *latency_us);
+
+gms_status gms_server_get_stats(gms_server *s, gms_server_stats *st);
+
+/* ------------------------------------------------------------------------ */
 /* Utilities                                                                 */
 /* ------------------------------------------------------------------------ */
 

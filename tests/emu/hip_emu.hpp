@@ -70,7 +70,8 @@ typedef enum hipError_t {
     hipSuccess = 0,
     hipErrorInvalidValue = 1,
     hipErrorOutOfMemory = 2,
-    hipErrorNoDevice = 100
+    hipErrorNoDevice = 100,
+    hipErrorNotReady = 600
 } hipError_t;
 typedef enum hipMemcpyKind {
     hipMemcpyHostToHost = 0,
@@ -125,6 +126,11 @@ inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t = nullptr) {
     return hipSuccess;
 }
 inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+// Emulated kernels finish inside their launch, so every recorded event is complete.
+inline hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
+#define hipEventDisableTiming 2u
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
     *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
     return hipSuccess;

@@ -116,7 +116,33 @@ def test_batch_autotune_keeps_results(dev):
     _check(items, want, "autotuned")
     st = b.stats()
     assert st["keys"] == sum(c[0] for c in MIXED)
-    assert st["launches"] == 1 + 3 + 2 * len({(max(1, int(np.ceil(np.log2(c[1])))), c[5]) for c in MIXED})
+    groups = len({(max(1, int(np.ceil(np.log2(c[1])))), c[5]) for c in MIXED})
+    assert st["launches"] == t["persistent"] + 3 + 2 * groups
+
+
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_batch_unaligned_instances(dev, persistent):
+    """Instances whose keys/values start off a 16-byte boundary take their
+    own group (scalar loads) next to aligned ones."""
+    import torch
+    banner(f"batch unaligned persistent={persistent}")
+    items, want = [], []
+    for i, (n, off) in enumerate([(40_000, 1), (6144 * 3, 0), (25_000, 3), (7, 2), (6145, 0)]):
+        keys = W.uniform_keys(n, seed=300 + i)
+        vals = np.arange(n, dtype=np.uint32)
+        kbuf = to_dev(np.concatenate([np.zeros(off, np.uint32), keys]), dev)
+        vbuf = to_dev(np.concatenate([np.zeros(off, np.uint32), vals]), dev)
+        kout = filled_u32(n + off, dev)
+        vout = filled_u32(n + off, dev)
+        it = gms.BatchItem(kbuf[off:], kout[off:], n, 64, gms.range_bucket(64), d_value_in=vbuf[off:],
+                           d_value_out=vout[off:], d_offsets=filled_u32(64, dev, 0xFFFFFFFF))
+        items.append(it)
+        want.append(oracle.multisplit(keys, 64, oracle.RANGE, 0, vals))
+    b = gms.Batch(items)
+    b.tuning = {"items_per_thread": 24, "persistent": persistent}
+    b.run()
+    torch.cuda.synchronize()
+    _check(items, want, "unaligned")
 
 
 def test_batch_validation(dev):

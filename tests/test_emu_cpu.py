@@ -178,3 +178,116 @@ def test_emulated_batch_rejects_overlapping_outputs(emu):
     bad = gms.gms_batch_item(p(a), None, p(b), None, 32, 300, gms.range_bucket(4).desc(), None)
     assert emu.gms_batch_plan(h, (gms.gms_batch_item * 1)(bad), 1, None) == 2
     emu.gms_batch_destroy(h)
+
+
+# ---------------------------------------------------------------------------
+# Request server on the emulation: mixed multisplit and sort requests, each
+# against the oracle; a budget that holds only some sorts at once (the rest
+# wait and are admitted as others finish); a request that can never fit.
+# ---------------------------------------------------------------------------
+def _emu_server(L, budget, max_inflight=2, max_step_keys=0):
+    cfg = gms.gms_server_config(budget, max_inflight, max_step_keys, 8)
+    h = ctypes.c_void_p()
+    assert L.gms_server_create(ctypes.byref(cfg), ctypes.byref(h)) == 0
+    return h
+
+
+def _server_requests(seed, count):
+    g = W.rng(seed)
+    reqs = []
+    for i in range(count):
+        n = int(g.integers(0, 9000))
+        keys = W.uniform_keys(n, seed=seed * 100 + i)
+        kv = bool(i % 2)
+        vals = np.arange(n, dtype=np.uint32)
+        kout = np.full(max(n, 1), 0x5A5A5A5A, np.uint32)
+        vout = np.full(max(n, 1), 0x5A5A5A5A, np.uint32)
+        if i % 3 == 2:
+            bits = (8, 7, 4)[i % 3 if i % 9 else 0]
+            req = gms.gms_request(1, p(keys), p(vals) if kv else None, p(kout), p(vout) if kv else None,
+                                  n, 0, gms.gms_bucket_desc(0, 0, 0), bits, None)
+            wk, wv = oracle.sort(keys, bits, vals if kv else None)
+            want = (wk, wv, None)
+            offs = None
+        else:
+            B = (2, 16, 64, 256)[i % 4]
+            offs = np.full(B, 0xFFFFFFFF, np.uint32)
+            req = gms.gms_request(0, p(keys), p(vals) if kv else None, p(kout), p(vout) if kv else None,
+                                  n, B, gms.range_bucket(B).desc(), 0, p(offs))
+            want = oracle.multisplit(keys, B, oracle.RANGE, 0, vals if kv else None)
+        reqs.append((req, (keys, vals, kout, vout, offs), want, n, kv))
+    return reqs
+
+
+def _check_server_results(reqs):
+    for i, (_req, (keys, vals, kout, vout, offs), (wk, wv, wo), n, kv) in enumerate(reqs):
+        np.testing.assert_array_equal(kout[:n], wk, err_msg=f"request {i} keys")
+        if kv:
+            np.testing.assert_array_equal(vout[:n], wv, err_msg=f"request {i} values")
+        if offs is not None:
+            np.testing.assert_array_equal(offs, wo, err_msg=f"request {i} offsets")
+
+
+@pytest.mark.parametrize("max_inflight", [1, 2])
+def test_emulated_server_mixed_requests(emu, max_inflight):
+    h = _emu_server(emu, 64 << 20, max_inflight)
+    try:
+        reqs = _server_requests(seed=max_inflight, count=18)
+        tickets = []
+        for req, *_ in reqs:
+            t = ctypes.c_uint64()
+            assert emu.gms_server_submit(h, ctypes.byref(req), ctypes.byref(t)) == 0
+            tickets.append(t.value)
+        for t in tickets:
+            lat = ctypes.c_double()
+            assert emu.gms_server_wait(h, t, ctypes.byref(lat)) == 0
+            assert lat.value > 0
+        st = gms.gms_server_stats()
+        assert emu.gms_server_get_stats(h, ctypes.byref(st)) == 0
+        assert st.completed == len(reqs) and st.submitted == len(reqs)
+        assert st.active == 0 and st.pending == 0
+        assert st.steps >= 4          # sorts take one step per digit
+    finally:
+        assert emu.gms_server_destroy(h) == 0
+    _check_server_results(reqs)
+
+
+def test_emulated_server_budget_admission(emu):
+    # 1 MiB budget: a 60000-key key-value sort reserves ~0.5 MiB + its step
+    # shares, so only one such sort is admitted at a time; a 200000-key sort
+    # can never fit and is rejected at submit.
+    h = _emu_server(emu, 1 << 20, 2, 1 << 16)
+    try:
+        n = 60000
+        bufs, tickets = [], []
+        for i in range(3):
+            keys = W.uniform_keys(n, seed=70 + i)
+            vals = np.arange(n, dtype=np.uint32)
+            kout = np.zeros(n, np.uint32)
+            vout = np.zeros(n, np.uint32)
+            bufs.append((keys, vals, kout, vout))
+            req = gms.gms_request(1, p(keys), p(vals), p(kout), p(vout), n, 0,
+                                  gms.gms_bucket_desc(0, 0, 0), 8, None)
+            t = ctypes.c_uint64()
+            assert emu.gms_server_submit(h, ctypes.byref(req), ctypes.byref(t)) == 0
+            tickets.append(t.value)
+        big = np.zeros(600000, np.uint32)
+        t = ctypes.c_uint64()
+        req = gms.gms_request(1, p(big), p(big), p(big) + 4 * 300000, None, 300000, 0,
+                              gms.gms_bucket_desc(0, 0, 0), 8, None)
+        assert emu.gms_server_submit(h, ctypes.byref(req), ctypes.byref(t)) == 1  # value in, no value out
+        req = gms.gms_request(1, p(big), None, p(big) + 4 * 300000, None, 300000, 0,
+                              gms.gms_bucket_desc(0, 0, 0), 8, None)
+        assert emu.gms_server_submit(h, ctypes.byref(req), ctypes.byref(t)) == 3  # never fits
+        for t in tickets:
+            assert emu.gms_server_wait(h, t, None) == 0
+        st = gms.gms_server_stats()
+        emu.gms_server_get_stats(h, ctypes.byref(st))
+        assert st.peak_scratch_bytes <= 1 << 20
+        assert st.rejected == 1 and st.completed == 3
+    finally:
+        emu.gms_server_destroy(h)
+    for keys, vals, kout, vout in bufs:
+        order = np.argsort(keys, kind="stable")
+        np.testing.assert_array_equal(kout, keys[order])
+        np.testing.assert_array_equal(vout, vals[order])
