@@ -259,51 +259,92 @@ def test_error_paths(dev):
     gms.multisplit_release_memory(ctx)
 
 
-@pytest.mark.slow
-@pytest.mark.parametrize("n,dist", [(1 << 27, "uniform"), (1 << 25, "uniform"), (1 << 25, "zipf"),
-                                    (1 << 25, "onehot90")])
-def test_full_size_256_kv_properties(dev, n, dist):
-    """2^25 / 2^27 keys, 256 buckets, key-value: complete property check on the GPU."""
+_KEY_CACHE = {}
+
+
+def _full_size_keys(dev, n, dist, B):
     import torch
-    B = 256
-    banner(f"full-size {dist} n={n}")
     if dist == "uniform":
         g = torch.Generator(device=dev)
         g.manual_seed(n)
-        kin = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=dev, generator=g)
-    else:
-        kin = to_dev(W.keys_for(dist, n, B, seed=3), dev)
+        return torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=dev, generator=g)
+    key = (n, dist, B)
+    if key not in _KEY_CACHE:
+        _KEY_CACHE.clear()
+        _KEY_CACHE[key] = W.keys_for(dist, n, B, seed=3)
+    return to_dev(_KEY_CACHE[key], dev)
+
+
+def _check_properties(dev, n, dist, B, kv):
+    """Complete property check of one full-size multisplit on the GPU: output
+    keys are a stable permutation of the input grouped by bucket, values
+    travel with their keys, offsets are the exclusive scan of the counts."""
+    import torch
+    banner(f"full-size {dist} n={n} B={B} kv={kv}")
+    kin = _full_size_keys(dev, n, dist, B)
     vin = torch.arange(n, dtype=torch.int32, device=dev)
     kout = torch.empty_like(kin)
     vout = torch.empty_like(vin)
     d_off = torch.zeros(B, dtype=torch.int32, device=dev)
     ctx = gms.multisplit_context(B)
     gms.multisplit_allocate_key_value(n, ctx)
-    gms.multisplit(kin, kout, n, ctx, gms.range_bucket(B), vin, vout, d_off)
+    if kv:
+        gms.multisplit(kin, kout, n, ctx, gms.range_bucket(B), vin, vout, d_off)
+    else:
+        gms.multisplit(kin, kout, n, ctx, gms.range_bucket(B), d_offsets=d_off)
     torch.cuda.synchronize()
     gms.multisplit_release_memory(ctx)
     ku = kin.to(torch.int64) & 0xFFFFFFFF
     bid_in = (ku * B) >> 32
     ko = kout.to(torch.int64) & 0xFFFFFFFF
     bid_out = (ko * B) >> 32
-    vo = vout.to(torch.int64)
-    # Range check first: the values index device tensors below, and an
-    # out-of-range index would fault instead of failing the test.
-    assert bool(((vo >= 0) & (vo < n)).all()), "output values outside [0, n)"
-    # values are a permutation of 0..n-1
-    seen = torch.zeros(n, dtype=torch.int32, device=dev)
-    seen.index_add_(0, vo, torch.ones(n, dtype=torch.int32, device=dev))
-    assert bool((seen == 1).all())
-    # keys travel with their values
-    assert bool((kout == kin[vo]).all())
-    # bucket ids non-decreasing; values increasing inside a bucket (stability)
+    # bucket ids non-decreasing
     assert bool((bid_out[1:] >= bid_out[:-1]).all())
-    same = bid_out[1:] == bid_out[:-1]
-    assert bool((vo[1:][same] > vo[:-1][same]).all())
     # offsets = exclusive scan of bucket counts
     cnt = torch.bincount(bid_in, minlength=B)
     want = torch.cumsum(cnt, 0) - cnt
     assert bool((d_off.to(torch.int64) == want).all())
+    if kv:
+        vo = vout.to(torch.int64)
+        # Range check first: the values index device tensors below, and an
+        # out-of-range index would fault instead of failing the test.
+        assert bool(((vo >= 0) & (vo < n)).all()), "output values outside [0, n)"
+        seen = torch.zeros(n, dtype=torch.int32, device=dev)
+        seen.index_add_(0, vo, torch.ones(n, dtype=torch.int32, device=dev))
+        assert bool((seen == 1).all())                      # a permutation of 0..n-1
+        assert bool((kout == kin[vo]).all())                 # keys travel with their values
+        same = bid_out[1:] == bid_out[:-1]
+        assert bool((vo[1:][same] > vo[:-1][same]).all())   # stable inside each bucket
+    else:
+        # key-only: inside each bucket the keys keep input order; compare with
+        # the stable order computed by torch (a sort by bucket id, stable)
+        order = torch.sort(bid_in, stable=True).indices
+        assert bool((kout == kin[order]).all())
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("n,dist", [(1 << 27, "uniform"), (1 << 25, "uniform"), (1 << 25, "zipf"),
+                                    (1 << 25, "onehot90")])
+def test_full_size_256_kv_properties(dev, n, dist):
+    """2^25 / 2^27 keys, 256 buckets, key-value."""
+    _check_properties(dev, n, dist, 256, True)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("dist", ["uniform", "zipf", "onehot90"])
+@pytest.mark.parametrize("B", [2, 8, 32, 64, 256])
+@pytest.mark.parametrize("kv", [False, True])
+def test_plain_sweep_2e25(dev, dist, B, kv):
+    """The plain-multisplit sweep at 2^25 keys: every bucket count and key
+    distribution, key-only and key-value."""
+    _check_properties(dev, 1 << 25, dist, B, kv)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("B", [2, 8, 32, 64])
+@pytest.mark.parametrize("kv", [False, True])
+def test_plain_sweep_2e27_uniform(dev, B, kv):
+    _check_properties(dev, 1 << 27, "uniform", B, kv)
 
 
 @pytest.mark.slow
