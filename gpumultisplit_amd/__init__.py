@@ -103,6 +103,7 @@ _SIGS = [
     ("gms_multisplit_key_value", _I, [_P, _P, _P, _P, _P, _U32, gms_bucket_desc, _P, _P]),
     ("gms_multisplit", _I, [_P, _P, _P, _P, _P, _U32, gms_bucket_desc, _P, _P]),
     ("gms_subproblem_size", _U32, [_U32, _I]),
+    ("gms_tile_keys", _U32, [_U32, _I, _U32]),
     ("gms_multisplit_set_profiling", _I, [_P, _I]),
     ("gms_multisplit_profile", _I, [_P, ctypes.POINTER(ctypes.c_double),
                                     ctypes.POINTER(ctypes.c_double),
@@ -305,6 +306,11 @@ def subproblem_size_bms_key_only(num_buckets: int) -> int:
 
 def subproblem_size_bms_key_value(num_buckets: int) -> int:
     return int(lib().gms_subproblem_size(num_buckets, 1))
+
+
+def tile_keys(num_buckets: int, key_value: bool, num_elements: int) -> int:
+    """Keys per tile of a call on num_elements keys."""
+    return int(lib().gms_tile_keys(num_buckets, int(bool(key_value)), int(num_elements)))
 
 
 class multisplit_context:

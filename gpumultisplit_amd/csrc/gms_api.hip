@@ -47,8 +47,8 @@ bool ranges_overlap(const void* a, size_t an, const void* b, size_t bn) {
 size_t matrix_elems(uint32_t n, uint32_t num_buckets) {
     if (num_buckets <= 1 || n == 0) return 0;
     const int lb = ceil_log2_u32(num_buckets);
-    const size_t t_ko = ((size_t)n + tile_keys(lb, false) - 1) / tile_keys(lb, false);
-    const size_t t_kv = ((size_t)n + tile_keys(lb, true) - 1) / tile_keys(lb, true);
+    const size_t t_ko = ((size_t)n + tile_keys(lb, false, 0) - 1) / tile_keys(lb, false, 0);
+    const size_t t_kv = ((size_t)n + tile_keys(lb, true, 0) - 1) / tile_keys(lb, true, 0);
     return ((size_t)1 << lb) * std::max(t_ko, t_kv);
 }
 
@@ -162,12 +162,13 @@ static bool vec_aligned(const void* p) { return p == nullptr || (((uintptr_t)p) 
 gms_status run_prescan_only(Scratch& s, const uint32_t* kin, uint32_t n, uint32_t B,
                             gms_bucket_desc d, bool kv, hipStream_t stream, uint32_t* nt_out) {
     const int lb = ceil_log2_u32(B);
-    const uint32_t tile = tile_keys(lb, kv);
+    const uint32_t tile = tile_keys(lb, kv, n);
     const uint32_t nt = (uint32_t)(((size_t)n + tile - 1) / tile);
     if (nt_out) *nt_out = nt;
     if (n == 0 || B <= 1) return GMS_SUCCESS;
     if (((size_t)1 << lb) * nt > s.hist_cap) return GMS_ERR_CAPACITY;
-    MsCall c{kin, nullptr, nullptr, nullptr, n, lb, kv, nt, s.hist, vec_aligned(kin), stream};
+    MsCall c{kin, nullptr, nullptr, nullptr, n, lb, kv, n >= kLongTileMin, nt, s.hist,
+             vec_aligned(kin), stream};
     return hip_status(with_functor(B, d, [&](auto f) { return launch_prescan(c, f); }));
 }
 
@@ -183,13 +184,13 @@ gms_status run_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin, 
         return hip_status(e);
     }
     const int lb = ceil_log2_u32(B);
-    const uint32_t tile = tile_keys(lb, kv);
+    const uint32_t tile = tile_keys(lb, kv, n);
     const uint32_t nt = (uint32_t)(((size_t)n + tile - 1) / tile);
     const size_t m = ((size_t)1 << lb) * nt;
     if (m > s.hist_cap || scan_partials_needed(m) > s.part_cap) return GMS_ERR_CAPACITY;
 
-    MsCall c{kin, vin, kout, vout, n, lb, kv, nt, s.hist, vec_aligned(kin) && vec_aligned(vin),
-             stream};
+    MsCall c{kin, vin, kout, vout, n, lb, kv, n >= kLongTileMin, nt, s.hist,
+             vec_aligned(kin) && vec_aligned(vin), stream};
     hipEvent_t* ev = (prof && prof->on) ? prof->next(stream) : nullptr;
     hipError_t e = hipSuccess;
     if (ev) e = hipEventRecord(ev[0], stream);
@@ -270,7 +271,12 @@ gms_status gms_bucket_ids_host(uint32_t num_buckets, gms_bucket_desc desc, const
 
 uint32_t gms_subproblem_size(uint32_t num_buckets, int key_value) {
     if (num_buckets < 1 || num_buckets > 256) return 0;
-    return tile_keys(ceil_log2_u32(num_buckets), key_value != 0);
+    return tile_keys(ceil_log2_u32(num_buckets), key_value != 0, 0);
+}
+
+uint32_t gms_tile_keys(uint32_t num_buckets, int key_value, uint32_t n) {
+    if (num_buckets < 1 || num_buckets > 256) return 0;
+    return tile_keys(ceil_log2_u32(num_buckets), key_value != 0, n);
 }
 
 // ---------------------------------------------------------------- multisplit

@@ -113,7 +113,7 @@ static int class_of(uint32_t num_buckets) {
 
 gms_status BatchEngine::plan(const gms_batch_item* items, uint32_t count, int slot) {
     const uint32_t K = tuning.items_per_thread;
-    if (K != 8 && K != 16 && K != 24) return GMS_ERR_INVALID_ARG;
+    if (K != 8 && K != 16 && K != 24 && K != 32) return GMS_ERR_INVALID_ARG;
     const uint32_t tile = (uint32_t)kThreads * K;
     slot_ = slot & 1;
     PinnedVec<BatchInst>& hi = h_inst_[slot_];
@@ -232,7 +232,8 @@ uint32_t BatchEngine::grid_for(const BatchGroup& g, bool post) const {
     switch (tuning.items_per_thread) {
         case 8: occ = batch_occupancy_k8(g.log_b, g.kv, post); break;
         case 16: occ = batch_occupancy_k16(g.log_b, g.kv, post); break;
-        default: occ = batch_occupancy_k24(g.log_b, g.kv, post); break;
+        case 24: occ = batch_occupancy_k24(g.log_b, g.kv, post); break;
+        default: occ = batch_occupancy_k32(g.log_b, g.kv, post); break;
     }
     // At least 8 workgroups (one per XCD) whenever there are 8 items, so that
     // every XCD's range of items has a workgroup to drain it.
@@ -256,7 +257,8 @@ gms_status BatchEngine::launch(const BatchDevice& d, hipStream_t s) {
         switch (tuning.items_per_thread) {
             case 8: return post ? launch_batch_postscan_k8(c) : launch_batch_prescan_k8(c);
             case 16: return post ? launch_batch_postscan_k16(c) : launch_batch_prescan_k16(c);
-            default: return post ? launch_batch_postscan_k24(c) : launch_batch_prescan_k24(c);
+            case 24: return post ? launch_batch_postscan_k24(c) : launch_batch_prescan_k24(c);
+            default: return post ? launch_batch_postscan_k32(c) : launch_batch_prescan_k32(c);
         }
     };
     for (uint32_t gi = 0; gi < G && e == hipSuccess; ++gi) e = one(groups[gi], gi, false);
@@ -382,7 +384,8 @@ gms_status gms_batch_launch_graph(gms_batch* b, void* stream) {
 
 gms_status gms_batch_set_tuning(gms_batch* b, gms_batch_tuning t) {
     if (!b) return GMS_ERR_INVALID_ARG;
-    if (t.items_per_thread != 8 && t.items_per_thread != 16 && t.items_per_thread != 24)
+    if (t.items_per_thread != 8 && t.items_per_thread != 16 && t.items_per_thread != 24 &&
+        t.items_per_thread != 32)
         return GMS_ERR_INVALID_ARG;
     if (t.persistent > 1) return GMS_ERR_INVALID_ARG;
     b->eng.tuning.items_per_thread = t.items_per_thread;
@@ -411,7 +414,7 @@ gms_status gms_batch_autotune(gms_batch* b, uint32_t reps, void* stream, gms_bat
     BatchTuning best = keep;
     float best_ms = -1.f;
     gms_status st = GMS_SUCCESS;
-    for (uint32_t K : {8u, 16u, 24u}) {
+    for (uint32_t K : {8u, 16u, 24u, 32u}) {
         for (uint32_t pers : {0u, 1u}) {
             b->eng.tuning.items_per_thread = K;
             b->eng.tuning.persistent = pers;

@@ -42,7 +42,7 @@ struct BatchGroup {
 };
 
 struct BatchTuning {
-    uint32_t items_per_thread = 24;  // 8, 16 or 24 keys per thread (tile = 256 x this)
+    uint32_t items_per_thread = 24;  // 8, 16, 24 or 32 keys per thread (tile = 256 x this)
     uint32_t persistent = 0;         // 0: one workgroup per tile; 1: resident grid + work queue
 };
 

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kThreads) void ms_batch_prescan_kernel(
     }
 }
 
-template <int LOG_B, int K, int LEVEL_BITS, bool KV, bool VEC>
+template <int LOG_B, int K, int LEVEL_BITS, bool KV, bool VEC, bool ALIAS>
 __device__ __forceinline__ void batch_postscan_item(
     const BatchInst* __restrict__ inst, const BatchWork* __restrict__ work, uint32_t w,
     const uint32_t* __restrict__ G, uint32_t m_total, uint32_t keys_total, uint32_t* s_keys,
@@ -135,7 +135,7 @@ __device__ __forceinline__ void batch_postscan_item(
 #pragma unroll
             for (int k = 0; k < K; ++k) val[k] = 0;
         }
-        postscan_rank<LOG_B, K, LEVEL_BITS, KV, true>(key, val, g0l, cntb, TILE, I.f, s_keys,
+        postscan_rank<LOG_B, K, LEVEL_BITS, KV, true, ALIAS>(key, val, g0l, cntb, TILE, I.f, s_keys,
                                                       s_vals, s_cnt, s_delta, s_bs, s_ws);
         postscan_write<K, KV, true>(TILE, I.n, I.kout, I.vout, I.f, s_keys, s_vals, s_delta);
     } else {
@@ -145,7 +145,7 @@ __device__ __forceinline__ void batch_postscan_item(
 #pragma unroll
             for (int k = 0; k < K; ++k) val[k] = 0;
         }
-        postscan_rank<LOG_B, K, LEVEL_BITS, KV, false>(key, val, g0l, cntb, nvalid, I.f, s_keys,
+        postscan_rank<LOG_B, K, LEVEL_BITS, KV, false, ALIAS>(key, val, g0l, cntb, nvalid, I.f, s_keys,
                                                        s_vals, s_cnt, s_delta, s_bs, s_ws);
         postscan_write<K, KV, false>(nvalid, I.n, I.kout, I.vout, I.f, s_keys, s_vals, s_delta);
     }
@@ -162,22 +162,26 @@ __global__ __launch_bounds__(kThreads) void ms_batch_postscan_kernel(
     static_assert(NLEV >= 1 && NLEV <= 3, "at most three ranking passes");
     static_assert(K % 4 == 0, "vector loads");
 
+    constexpr uint32_t CNT = (1u << LEVEL_BITS) * kThreads;
+    constexpr bool ALIAS = tune_alias(LOG_B, K, KV, LEVEL_BITS);
     __shared__ __attribute__((aligned(16))) uint32_t s_keys[TILE];
-    __shared__ __attribute__((aligned(16))) uint32_t s_vals[KV ? TILE : 4];
-    __shared__ __attribute__((aligned(16))) uint32_t s_cnt[(1 << LEVEL_BITS) * kThreads];
+    __shared__ __attribute__((aligned(16))) uint32_t s_vc[KV ? TILE : CNT];
+    __shared__ __attribute__((aligned(16))) uint32_t s_cnt_sep[(KV && !ALIAS) ? CNT : 4];
     __shared__ uint32_t s_delta[B];
     __shared__ uint32_t s_bs[kWaves];
     __shared__ uint32_t s_ws[kWaves];
+    uint32_t* s_vals = s_vc;
+    uint32_t* s_cnt = KV ? (ALIAS ? s_vc : s_cnt_sep) : s_vc;
 
     if constexpr (!PERSIST) {
-        batch_postscan_item<LOG_B, K, LEVEL_BITS, KV, VEC>(
+        batch_postscan_item<LOG_B, K, LEVEL_BITS, KV, VEC, ALIAS>(
             inst, work, xcd_tile(blockIdx.x, num_work), G, m_total, keys_total, s_keys, s_vals,
             s_cnt, s_delta, s_bs, s_ws);
     } else {
         __shared__ uint32_t s_next;
         const XcdRange R = xcd_range(num_work);
         for (uint32_t i = blockIdx.x / 8u; i < R.count; i = xcd_next(queue, R.wgs, &s_next))
-            batch_postscan_item<LOG_B, K, LEVEL_BITS, KV, VEC>(inst, work, R.begin + i, G,
+            batch_postscan_item<LOG_B, K, LEVEL_BITS, KV, VEC, ALIAS>(inst, work, R.begin + i, G,
                                                                 m_total, keys_total, s_keys,
                                                                 s_vals, s_cnt, s_delta, s_bs,
                                                                 s_ws);
@@ -204,13 +208,16 @@ struct BatchLaunch {
 hipError_t launch_batch_prescan_k8(const BatchLaunch& c);
 hipError_t launch_batch_prescan_k16(const BatchLaunch& c);
 hipError_t launch_batch_prescan_k24(const BatchLaunch& c);
+hipError_t launch_batch_prescan_k32(const BatchLaunch& c);
 hipError_t launch_batch_postscan_k8(const BatchLaunch& c);
 hipError_t launch_batch_postscan_k16(const BatchLaunch& c);
 hipError_t launch_batch_postscan_k24(const BatchLaunch& c);
+hipError_t launch_batch_postscan_k32(const BatchLaunch& c);
 // Resident workgroups per CU of the persistent batch kernels.
 int batch_occupancy_k8(int log_b, bool kv, bool postscan);
 int batch_occupancy_k16(int log_b, bool kv, bool postscan);
 int batch_occupancy_k24(int log_b, bool kv, bool postscan);
+int batch_occupancy_k32(int log_b, bool kv, bool postscan);
 
 template <int K>
 struct BatchKernels {

@@ -177,8 +177,13 @@ __global__ __launch_bounds__(kThreads) void ms_prescan_kernel(const uint32_t* __
 // keys of thread t.  Counting hits 64 distinct banks per wave; the scan reads
 // D consecutive words per thread, so position = exclusive prefix in memory
 // order = (keys of smaller digits) + (keys of this digit in earlier threads).
+//
+// ALIAS: s_cnt lies inside s_vals (key-value tiles), which keeps the LDS of a
+// 256-bucket key-value workgroup under a third of the CU's 160 KB.  Every
+// thread then reads all its positions into registers, and a barrier separates
+// those reads from the scatter that overwrites the counters.
 // ---------------------------------------------------------------------------
-template <int SHIFT, int W, int K, bool KV, bool FULL, class F>
+template <int SHIFT, int W, int K, bool KV, bool FULL, bool ALIAS, class F>
 __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32_t (&val)[K],
                                           uint32_t nvalid, F f, uint32_t* s_keys,
                                           uint32_t* s_vals, uint32_t* s_cnt, uint32_t* s_ws) {
@@ -241,13 +246,29 @@ __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32
     }
     __syncthreads();
 
+    if constexpr (ALIAS) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        if (FULL || first + (uint32_t)k < nvalid) {
-            uint32_t pos = s_cnt[(dg[k] << 8) | tid] + lr[k];
-            pos = pos < nvalid ? pos : nvalid - 1u;  // in bounds even if counts were inconsistent
-            s_keys[pos] = key[k];
-            if constexpr (KV) s_vals[pos] = val[k];
+        for (int k = 0; k < K; ++k) {
+            const uint32_t pos = s_cnt[(dg[k] << 8) | tid] + lr[k];
+            lr[k] = pos < nvalid ? pos : nvalid - 1u;  // in bounds even if counts were inconsistent
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (FULL || first + (uint32_t)k < nvalid) {
+                s_keys[lr[k]] = key[k];
+                if constexpr (KV) s_vals[lr[k]] = val[k];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (FULL || first + (uint32_t)k < nvalid) {
+                uint32_t pos = s_cnt[(dg[k] << 8) | tid] + lr[k];
+                pos = pos < nvalid ? pos : nvalid - 1u;  // in bounds even if counts were inconsistent
+                s_keys[pos] = key[k];
+                if constexpr (KV) s_vals[pos] = val[k];
+            }
         }
     }
     __syncthreads();
@@ -274,7 +295,7 @@ __device__ __forceinline__ void reload_blocked(uint32_t (&key)[K], uint32_t (&va
 // In-tile ranking of one tile: delta[b] = global start - local start of every
 // bucket run, then the stable LDS passes; leaves the tile in bucket order in
 // s_keys/s_vals.
-template <int LOG_B, int K, int LEVEL_BITS, bool KV, bool FULL, class F>
+template <int LOG_B, int K, int LEVEL_BITS, bool KV, bool FULL, bool ALIAS, class F>
 __device__ __forceinline__ void postscan_rank(uint32_t (&key)[K], uint32_t (&val)[K], uint32_t g0,
                                               uint32_t cntb, uint32_t nvalid, F f,
                                               uint32_t* s_keys, uint32_t* s_vals,
@@ -289,18 +310,20 @@ __device__ __forceinline__ void postscan_rank(uint32_t (&key)[K], uint32_t (&val
         if (tid < (uint32_t)B) s_delta[tid] = g0 - ls;
     }
     constexpr int W0 = LOG_B < LEVEL_BITS ? LOG_B : LEVEL_BITS;
-    rank_pass<0, W0, K, KV, FULL>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_ws);
+    rank_pass<0, W0, K, KV, FULL, ALIAS>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_ws);
     if constexpr (NLEV >= 2) {
         constexpr int S1 = LEVEL_BITS;
         constexpr int W1 = (LOG_B - S1) < LEVEL_BITS ? (LOG_B - S1) : LEVEL_BITS;
         reload_blocked<K, KV>(key, val, s_keys, s_vals);
-        rank_pass<S1, W1, K, KV, FULL>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_ws);
+        if constexpr (ALIAS) __syncthreads();  // values read before the counters are cleared
+        rank_pass<S1, W1, K, KV, FULL, ALIAS>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_ws);
     }
     if constexpr (NLEV >= 3) {
         constexpr int S2 = 2 * LEVEL_BITS;
         constexpr int W2 = LOG_B - S2;
         reload_blocked<K, KV>(key, val, s_keys, s_vals);
-        rank_pass<S2, W2, K, KV, FULL>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_ws);
+        if constexpr (ALIAS) __syncthreads();
+        rank_pass<S2, W2, K, KV, FULL, ALIAS>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_ws);
     }
 }
 
@@ -339,12 +362,18 @@ __global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
     static_assert(NLEV >= 1 && NLEV <= 3, "at most three ranking passes");
     static_assert(K % 4 == 0, "vector loads");
 
+    // Key-value tiles keep the digit counters inside the value staging area
+    // (ALIAS in rank_pass); key-only tiles have no values and a separate area.
+    constexpr uint32_t CNT = (1u << LEVEL_BITS) * kThreads;
+    constexpr bool ALIAS = tune_alias(LOG_B, K, KV, LEVEL_BITS);
     __shared__ __attribute__((aligned(16))) uint32_t s_keys[TILE];
-    __shared__ __attribute__((aligned(16))) uint32_t s_vals[KV ? TILE : 4];
-    __shared__ __attribute__((aligned(16))) uint32_t s_cnt[(1 << LEVEL_BITS) * kThreads];
+    __shared__ __attribute__((aligned(16))) uint32_t s_vc[KV ? TILE : CNT];
+    __shared__ __attribute__((aligned(16))) uint32_t s_cnt_sep[(KV && !ALIAS) ? CNT : 4];
     __shared__ uint32_t s_delta[B];
     __shared__ uint32_t s_bs[kWaves];
     __shared__ uint32_t s_ws[kWaves];
+    uint32_t* s_vals = s_vc;
+    uint32_t* s_cnt = KV ? (ALIAS ? s_vc : s_cnt_sep) : s_vc;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t t = xcd_tile(blockIdx.x, num_tiles);
@@ -369,7 +398,7 @@ __global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
             for (int k = 0; k < K; ++k) val[k] = 0;
         }
         const uint32_t cntb = (idx + 1 < mtot ? g1raw : n) - g0;
-        postscan_rank<LOG_B, K, LEVEL_BITS, KV, true>(key, val, g0, cntb, TILE, f, s_keys,
+        postscan_rank<LOG_B, K, LEVEL_BITS, KV, true, ALIAS>(key, val, g0, cntb, TILE, f, s_keys,
                                                       s_vals, s_cnt, s_delta, s_bs, s_ws);
         postscan_write<K, KV, true>(TILE, n, kout, vout, f, s_keys, s_vals, s_delta);
     } else {
@@ -380,7 +409,7 @@ __global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
             for (int k = 0; k < K; ++k) val[k] = 0;
         }
         const uint32_t cntb = (idx + 1 < mtot ? g1raw : n) - g0;
-        postscan_rank<LOG_B, K, LEVEL_BITS, KV, false>(key, val, g0, cntb, nvalid, f, s_keys,
+        postscan_rank<LOG_B, K, LEVEL_BITS, KV, false, ALIAS>(key, val, g0, cntb, nvalid, f, s_keys,
                                                        s_vals, s_cnt, s_delta, s_bs, s_ws);
         postscan_write<K, KV, false>(nvalid, n, kout, vout, f, s_keys, s_vals, s_delta);
     }

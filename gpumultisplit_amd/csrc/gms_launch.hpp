@@ -6,23 +6,29 @@
 
 namespace gms {
 
-template <int LOG_B, bool KV, bool VEC, class F>
+template <int LOG_B, bool KV, bool VEC, bool LARGE, class F>
 hipError_t launch_prescan_v(const MsCall& c, const F& f) {
-    constexpr int K = tune_items(LOG_B, KV);
+    constexpr int K = tune_items(LOG_B, KV, LARGE);
     hipLaunchKernelGGL((ms_prescan_kernel<LOG_B, K, VEC, F>), dim3(c.num_tiles), dim3(kThreads),
                        0, c.stream, c.kin, c.n, c.num_tiles, c.H, f);
     return hipGetLastError();
 }
 
+// The long-tile shape is instantiated only for classes where it differs.
 template <int LOG_B, bool KV, class F>
 hipError_t launch_prescan_t(const MsCall& c, const F& f) {
-    return c.vec_ok ? launch_prescan_v<LOG_B, KV, true>(c, f)
-                    : launch_prescan_v<LOG_B, KV, false>(c, f);
+    if constexpr (tune_items(LOG_B, KV, true) != tune_items(LOG_B, KV, false)) {
+        if (c.large)
+            return c.vec_ok ? launch_prescan_v<LOG_B, KV, true, true>(c, f)
+                            : launch_prescan_v<LOG_B, KV, false, true>(c, f);
+    }
+    return c.vec_ok ? launch_prescan_v<LOG_B, KV, true, false>(c, f)
+                    : launch_prescan_v<LOG_B, KV, false, false>(c, f);
 }
 
-template <int LOG_B, bool KV, bool VEC, class F>
+template <int LOG_B, bool KV, bool VEC, bool LARGE, class F>
 hipError_t launch_postscan_v(const MsCall& c, const F& f) {
-    constexpr int K = tune_items(LOG_B, KV);
+    constexpr int K = tune_items(LOG_B, KV, LARGE);
     constexpr int LB = tune_level_bits(LOG_B);
     hipLaunchKernelGGL((ms_postscan_kernel<LOG_B, K, LB, KV, VEC, F>), dim3(c.num_tiles),
                        dim3(kThreads), 0, c.stream, c.kin, c.vin, c.kout, c.vout, c.n,
@@ -32,8 +38,13 @@ hipError_t launch_postscan_v(const MsCall& c, const F& f) {
 
 template <int LOG_B, bool KV, class F>
 hipError_t launch_postscan_t(const MsCall& c, const F& f) {
-    return c.vec_ok ? launch_postscan_v<LOG_B, KV, true>(c, f)
-                    : launch_postscan_v<LOG_B, KV, false>(c, f);
+    if constexpr (tune_items(LOG_B, KV, true) != tune_items(LOG_B, KV, false)) {
+        if (c.large)
+            return c.vec_ok ? launch_postscan_v<LOG_B, KV, true, true>(c, f)
+                            : launch_postscan_v<LOG_B, KV, false, true>(c, f);
+    }
+    return c.vec_ok ? launch_postscan_v<LOG_B, KV, true, false>(c, f)
+                    : launch_postscan_v<LOG_B, KV, false, false>(c, f);
 }
 
 #define GMS_SWITCH_LOGB_KV(FN, c, f)                                   \

@@ -334,7 +334,7 @@ gms_status gms_server_create(const gms_server_config* cfg, gms_server** out) {
     if (cfg->scratch_budget_bytes < (1u << 20)) return GMS_ERR_INVALID_ARG;
     if (cfg->max_inflight_steps < 1 || cfg->max_inflight_steps > 2) return GMS_ERR_INVALID_ARG;
     const uint32_t K = cfg->items_per_thread ? cfg->items_per_thread : 24;
-    if (K != 8 && K != 16 && K != 24) return GMS_ERR_INVALID_ARG;
+    if (K != 8 && K != 16 && K != 24 && K != 32) return GMS_ERR_INVALID_ARG;
     gms_server* s = new (std::nothrow) gms_server();
     if (!s) return GMS_ERR_ALLOC;
     s->cfg = *cfg;

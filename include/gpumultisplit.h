@@ -119,8 +119,14 @@ gms_status gms_multisplit(gms_multisplit_ctx *ctx, const uint32_t *d_key_in,
                           uint32_t *d_offsets, void *stream);
 
 /* subproblem_size_bms_key_only / _key_value (src/api/bms_api.h:23-89): keys
- * per tile for a bucket count.  Returns 0 for an unsupported count. */
+ * per tile for a bucket count (the tile of inputs below 2^26 keys, also the
+ * shortest tile of any input).  Returns 0 for an unsupported count. */
 uint32_t gms_subproblem_size(uint32_t num_buckets, int key_value);
+
+/* Keys per tile of an n-key call (inputs of 2^26 keys and more may use a
+ * longer tile); no counterpart in the reference, whose tile depends on the
+ * bucket count only.  Returns 0 for an unsupported count. */
+uint32_t gms_tile_keys(uint32_t num_buckets, int key_value, uint32_t n);
 
 /* Stage timing for measurement: when enabled, every multisplit on this context
  * records HIP events on its stream around the prescan, scan and postscan
@@ -133,7 +139,7 @@ gms_status gms_multisplit_profile(gms_multisplit_ctx *ctx, double *ms_prescan,
 
 /* Stage-level check: runs only the prescan on d_key_in and copies the per-tile
  * bucket counts, bucket-major (H[b * tiles + t]), to d_matrix (B * tiles u32,
- * tiles = ceil(n / gms_subproblem_size)).  Same quantity the reference's
+ * tiles = ceil(n / gms_tile_keys(B, key_value, n))).  Same quantity the reference's
  * cpu_prescan_test checks (src/cpu_functions.h:55-103). */
 gms_status gms_multisplit_prescan_matrix(gms_multisplit_ctx *ctx, const uint32_t *d_key_in,
                                          uint32_t n, gms_bucket_desc desc, int key_value,
@@ -240,7 +246,7 @@ typedef struct gms_batch_item {
 
 /* Tile shape and grid of the batched kernels (picked by gms_batch_autotune). */
 typedef struct gms_batch_tuning {
-    uint32_t items_per_thread;  /* 8, 16 or 24: tile = 256 x items_per_thread keys */
+    uint32_t items_per_thread;  /* 8, 16, 24 or 32: tile = 256 x items_per_thread keys */
     uint32_t persistent;        /* 0: one workgroup per tile; 1: resident grid
                                    pulling tiles from a device work queue */
 } gms_batch_tuning;
@@ -308,7 +314,7 @@ typedef struct gms_server_config {
     uint64_t scratch_budget_bytes;  /* device scratch arena; admission is bounded by it */
     uint32_t max_inflight_steps;    /* steps queued on the GPU at once (1 or 2) */
     uint32_t max_step_keys;         /* keys per step (0 = no limit below 2^31) */
-    uint32_t items_per_thread;      /* batched-kernel tile shape: 8, 16 or 24 */
+    uint32_t items_per_thread;      /* batched-kernel tile shape: 8, 16, 24 or 32 */
 } gms_server_config;
 
 typedef struct gms_server_stats {

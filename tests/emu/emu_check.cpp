@@ -209,7 +209,7 @@ int main(int argc, char** argv) {
     printf("compaction/histogram done\n");
 
     // batched multisplit: exact-size buffers per instance, every tuning
-    for (uint32_t K : {8u, 16u, 24u}) {
+    for (uint32_t K : {8u, 16u, 24u, 32u}) {
         for (uint32_t pers : {0u, 1u}) {
             const uint32_t ns[] = {0, 1, 777, 2048, 4096 + 3, 6144, 13000};
             const uint32_t nbs[] = {8, 2, 1, 100, 64, 256, 17};

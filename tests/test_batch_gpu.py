@@ -70,7 +70,7 @@ def _clear(items):
             it.d_value_out.fill_(0x5A5A5A5A)
 
 
-@pytest.mark.parametrize("tuning", [(8, 0), (8, 1), (16, 0), (16, 1), (24, 0), (24, 1)])
+@pytest.mark.parametrize("tuning", [(8, 0), (8, 1), (16, 0), (16, 1), (24, 0), (24, 1), (32, 0), (32, 1)])
 def test_batch_mixed_matches_oracle(dev, tuning):
     import torch
     banner(f"batch mixed tuning={tuning}")
@@ -107,7 +107,7 @@ def test_batch_autotune_keeps_results(dev):
     items, want = _make(dev, MIXED, seed=5)
     b = gms.Batch(items)
     t = b.autotune(reps=2)
-    assert t["items_per_thread"] in (8, 16, 24) and t["persistent"] in (0, 1)
+    assert t["items_per_thread"] in (8, 16, 24, 32) and t["persistent"] in (0, 1)
     assert t["ms_per_run"] > 0
     assert b.tuning == {"items_per_thread": t["items_per_thread"], "persistent": t["persistent"]}
     _clear(items)

@@ -158,7 +158,7 @@ def _emu_batch(L, cases, tuning, seed):
             np.testing.assert_array_equal(vout[:n], wv, err_msg=f"instance {i} values")
 
 
-@pytest.mark.parametrize("tuning", [(8, 0), (8, 1), (16, 1), (24, 0)])
+@pytest.mark.parametrize("tuning", [(8, 0), (8, 1), (16, 1), (24, 0), (32, 1)])
 def test_emulated_batch_matches_oracle(emu, tuning):
     _emu_batch(emu, BATCH_CASES, tuning, seed=sum(tuning))
 

@@ -206,14 +206,16 @@ def test_device_offsets_and_determinism(dev):
 
 def test_prescan_matrix_stage(dev):
     import torch
-    for B, n in ((256, 100_000), (8, 50_001), (3, 9000)):
+    for B, n, kv in ((256, 100_000, True), (8, 50_001, True), (3, 9000, True),
+                     (256, (1 << 26) + 5, False), (64, (1 << 26) + 3, True)):
         ctx = gms.multisplit_context(B)
         gms.multisplit_allocate_key_value(n, ctx)
         keys = W.uniform_keys(n, seed=B)
-        tile = gms.subproblem_size_bms_key_value(B)
+        tile = gms.tile_keys(B, kv, n)
+        assert tile >= gms.subproblem_size_bms_key_value(B)
         nt = (n + tile - 1) // tile
         mat = torch.zeros(B * nt, dtype=torch.int32, device=dev)
-        gms.prescan_matrix(to_dev(keys, dev), n, ctx, gms.range_bucket(B), mat, key_value=True)
+        gms.prescan_matrix(to_dev(keys, dev), n, ctx, gms.range_bucket(B), mat, key_value=kv)
         np.testing.assert_array_equal(from_dev(mat), oracle.tile_histograms(keys, B, tile))
         gms.multisplit_release_memory(ctx)
 
@@ -341,10 +343,23 @@ def test_plain_sweep_2e25(dev, dist, B, kv):
 
 
 @pytest.mark.slow
-@pytest.mark.parametrize("B", [2, 8, 32, 64])
+@pytest.mark.parametrize("B", [2, 8, 32, 64, 256])
 @pytest.mark.parametrize("kv", [False, True])
 def test_plain_sweep_2e27_uniform(dev, B, kv):
     _check_properties(dev, 1 << 27, "uniform", B, kv)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("B,kv,extra", [(128, False, 12345), (64, True, 777), (256, False, 1),
+                                        (256, True, 6143)])
+def test_long_tiles_against_oracle(dev, B, kv, extra):
+    """Inputs of 2^26 keys and more take the long-tile shape of some classes;
+    ragged sizes there, bit-exact against the oracle."""
+    n = (1 << 26) + extra
+    banner(f"long tiles B={B} kv={kv} n={n} tile={gms.tile_keys(B, kv, n)}")
+    keys = W.uniform_keys(n, seed=B + extra)
+    got = run_ms(dev, keys, B, gms.range_bucket(B), kv)
+    check_equal(got, oracle_ms(keys, B, oracle.RANGE), kv)
 
 
 @pytest.mark.slow

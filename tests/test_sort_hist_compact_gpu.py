@@ -106,6 +106,22 @@ def test_histogram_u32_bins(dev, num_bins):
         np.testing.assert_array_equal(from_dev(h), oracle.histogram(keys, num_bins, oracle.RANGE))
 
 
+def test_histogram_unaligned_and_tails(dev):
+    import torch
+    for n, off in ((4096 * 3 + 17, 1), (4096 * 512 * 2 + 5, 3), (100, 2), (4096 * 600, 0)):
+        keys = W.uniform_keys(n, seed=n + off)
+        buf = to_dev(np.concatenate([np.zeros(off, np.uint32), keys]), dev)
+        h = torch.zeros(256, dtype=torch.int32, device=dev)
+        gms.histogram(buf[off:], n, h, 256, gms.ms_bucket_sort_8bit(8))
+        np.testing.assert_array_equal(from_dev(h), oracle.histogram(keys, 256, oracle.DIGIT, 8))
+        x = (W.rng(n).random(n, dtype=np.float32) * np.float32(300.0) - np.float32(20.0)).astype(np.float32)
+        xb = torch.from_numpy(np.concatenate([np.zeros(off, np.float32), x])).to(dev)
+        he = torch.zeros(64, dtype=torch.int32, device=dev)
+        inv = float(np.float32(64) / np.float32(256))
+        gms.multisplit2_histogram_even(xb[off:], n, he, 0.0, 256.0, inv, 64)
+        np.testing.assert_array_equal(from_dev(he).view(np.int32), oracle.histogram_even_f32(x, 64, 0.0, inv))
+
+
 def test_histogram_256_low_byte_accumulates(dev):
     import torch
     # 256-bin histogram of the low 8 bits, added twice into the same array
