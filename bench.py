@@ -141,13 +141,18 @@ def bench_batched(args, torch, gms, W, dev):
     sizes = W.log_uniform_sizes(count, 8, 22, seed=4096)
     total = int(sizes.sum())
     print(f"[bench] batched: {count} instances, {total} keys", file=sys.stderr, flush=True)
+    # instances packed into one allocation, each starting on a 16-byte boundary
+    # (as separately allocated buffers would)
+    starts = np.zeros(count, dtype=np.int64)
+    for i in range(1, count):
+        starts[i] = (starts[i - 1] + sizes[i - 1] + 3) // 4 * 4
+    span = int(starts[-1] + sizes[-1])
     g = torch.Generator(device=dev)
     g.manual_seed(4096)
-    kin = torch.randint(-(1 << 31), 1 << 31, (total,), dtype=torch.int32, device=dev, generator=g)
-    vin = torch.arange(total, dtype=torch.int32, device=dev)
+    kin = torch.randint(-(1 << 31), 1 << 31, (span,), dtype=torch.int32, device=dev, generator=g)
+    vin = torch.arange(span, dtype=torch.int32, device=dev)
     kout = torch.empty_like(kin)
     vout = torch.empty_like(kin)
-    starts = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
     f = gms.range_bucket(B)
     items = [gms.BatchItem(kin[s:s + n], kout[s:s + n], int(n), B, f, d_value_in=vin[s:s + n],
                            d_value_out=vout[s:s + n]) for s, n in zip(starts, sizes)]
@@ -188,7 +193,8 @@ def bench_batched(args, torch, gms, W, dev):
     st = batch.stats()
     del batch, items
     return {
-        "workload": f"{count} key-value multisplits, sizes round(2^U) U~U[8,22], {B} buckets (range)",
+        "workload": f"{count} key-value multisplits, sizes round(2^U) U~U[8,22], {B} buckets (range), "
+                    "16-byte-aligned instance buffers",
         "total_keys": total, "instances": count,
         "ms_per_batch": round(ms_batch, 4), "gkeys_per_s": round(total / ms_batch / 1e6, 3),
         "plain_back_to_back_ms": round(ms_plain, 4),
@@ -199,6 +205,86 @@ def bench_batched(args, torch, gms, W, dev):
         "tuning": {"items_per_thread": tuned["items_per_thread"], "persistent": tuned["persistent"]},
         "hipgraph": True, "reps": reps,
     }
+
+
+def bench_aux(args, torch, gms, W, dev):
+    """The other hot-path rows, one JSON line each (measurement runs):
+    sort{7,8}_{kv,ko}: radix sort of 2^25 uint32 keys (value = index);
+    hist256: 256-bin histogram of the low 8 bits of 2^27 keys;
+    compact{50,5}: 2-bucket compaction of 2^27 keys, predicate key < p*2^32;
+    copy: the streaming copy kernel over 2^27 u32 (the measured copy ceiling)."""
+    import numpy as np
+    w = args.workload
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    L = gms.lib()
+    g = torch.Generator(device=dev)
+    g.manual_seed(12345)
+    if w.startswith("sort"):
+        bits, kv = int(w[4]), w.endswith("kv")
+        n = 1 << 25
+        passes = {8: 4, 7: 5}[bits]
+        bpk = passes * (20 if kv else 12)
+        metric = f"{'key-value' if kv else 'key-only'} radix sort ({bits}-bit digits), {n} uint32 keys"
+    elif w == "hist256":
+        n, bpk, metric = 1 << 27, 4, "256-bin histogram of the low 8 bits, 134217728 uint32 keys"
+    elif w.startswith("compact"):
+        pct = int(w[7:])
+        n, bpk = 1 << 27, 12
+        metric = f"2-bucket compaction ({pct}% kept), 134217728 uint32 keys"
+    elif w == "copy":
+        n, bpk, metric = 1 << 27, 8, "device copy of 134217728 uint32"
+    else:
+        raise SystemExit(f"unknown workload {w!r}")
+    kin = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=dev, generator=g)
+    kout = torch.empty_like(kin)
+    if w.startswith("sort"):
+        vin = torch.arange(n, dtype=torch.int32, device=dev) if kv else None
+        vout = torch.empty_like(kin) if kv else None
+        ctx = gms.ms_sort_context()
+        L.gms_sort_allocate(ctx.handle, n, bits, int(kv))
+        vp, vop = (vin.data_ptr(), vout.data_ptr()) if kv else (None, None)
+
+        def step():
+            st = L.gms_sort(ctx.handle, kin.data_ptr(), kout.data_ptr(), vp, vop, n, bits, stream)
+            if st != 0:
+                raise gms.GmsError(st, "gms_sort")
+    elif w == "hist256":
+        h = torch.zeros(256, dtype=torch.int32, device=dev)
+        desc = gms.ms_bucket_sort_8bit(0).desc()
+
+        def step():
+            st = L.gms_histogram_u32(kin.data_ptr(), n, h.data_ptr(), 256, desc, stream)
+            if st != 0:
+                raise gms.GmsError(st, "gms_histogram_u32")
+    elif w.startswith("compact"):
+        ctx = gms.compaction_context()
+        gms.compaction_allocate_key_only(n, ctx)
+        desc = gms.delta_func(int(pct / 100 * (1 << 32))).desc()
+        cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+
+        def step():
+            st = L.gms_compaction(ctx.handle, kin.data_ptr(), None, kout.data_ptr(), None, n, desc,
+                                  None, cnt.data_ptr(), stream)
+            if st != 0:
+                raise gms.GmsError(st, "gms_compaction")
+    else:
+        def step():
+            gms.copy_u32(kin, kout, n)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    out = {"metric": metric, "value": round(n / ms / 1e6, 3), "unit": "Gkeys/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+           "data": "synthetic (uniform keys)", "config": {"workload": w, "n": n},
+           "call_hbm_fraction": round(n * bpk / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "algorithmic_bytes_per_key": bpk}
+    print(json.dumps(out), flush=True)
 
 
 def traffic_from_profiles(workload):
@@ -247,7 +333,8 @@ def main():
     stream = torch.cuda.current_stream(dev).cuda_stream
     cfg = parse_workload(args.workload)
     if cfg["kind"] != "ms":
-        raise SystemExit(f"workload {args.workload!r} not available in this bench version")
+        bench_aux(args, torch, gms, W, dev)
+        return
 
     n = 1 << cfg["log2n"]
     B = cfg["B"]

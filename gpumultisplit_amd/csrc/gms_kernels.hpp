@@ -420,12 +420,14 @@ __global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
 // ---------------------------------------------------------------------------
 constexpr int kScanItems = 16;
 constexpr uint32_t kScanChunk = kThreads * kScanItems;  // 4096 elements per workgroup
+constexpr uint32_t kScanFusedParts = 2048;  // chunks up to which the partials pass is folded
 
 __global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const uint32_t* __restrict__ in,
                                                                uint32_t m,
                                                                uint32_t* __restrict__ part);
 __global__ __launch_bounds__(1024) void scan_partials_kernel(uint32_t* __restrict__ part,
                                                              uint32_t np);
+template <bool SUM_PARTS>
 __global__ __launch_bounds__(kThreads) void scan_down_kernel(const uint32_t* in, uint32_t* out,
                                                              uint32_t m,
                                                              const uint32_t* __restrict__ part);

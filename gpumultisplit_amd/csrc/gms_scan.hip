@@ -3,8 +3,10 @@
 // The scan replaces cub::DeviceScan::ExclusiveSum of the reference's scan
 // stage (src/api/multisplit.cuh:141, :258).  Reduce-then-scan over chunks of
 // 4096 elements: (1) each workgroup sums its chunk, (2) one workgroup scans the
-// chunk sums, (3) each workgroup rescans its chunk with its offset.  Chunks
-// are read with coalesced 16-byte loads.
+// chunk sums, (3) each workgroup rescans its chunk with its offset.  Up to
+// kScanFusedParts chunks, step (2) is folded into (3): every workgroup adds
+// the sums of the chunks before its own.  Chunks are read with coalesced
+// 16-byte loads.
 #include "gms_internal.hpp"
 #include "gms_kernels.hpp"
 
@@ -48,6 +50,11 @@ __global__ __launch_bounds__(1024) void scan_partials_kernel(uint32_t* __restric
     }
 }
 
+// Downsweep of chunk blockIdx.x.  SUM_PARTS: `part` holds the unscanned
+// chunk sums and the workgroup adds up those before its own (used when there
+// are at most kScanFusedParts chunks, which saves the partials launch);
+// otherwise `part` is already an exclusive scan of them.
+template <bool SUM_PARTS>
 __global__ __launch_bounds__(kThreads) void scan_down_kernel(const uint32_t* in, uint32_t* out,
                                                              uint32_t m,
                                                              const uint32_t* __restrict__ part) {
@@ -67,11 +74,22 @@ __global__ __launch_bounds__(kThreads) void scan_down_kernel(const uint32_t* in,
 #pragma unroll
         for (int i = 0; i < kScanItems; ++i) x[i] = (base + off + i < m) ? in[base + off + i] : 0u;
     }
+    uint32_t base_sum;
+    if constexpr (SUM_PARTS) {
+        __shared__ uint32_t s_p[kWaves];
+        uint32_t ps = 0;
+        for (uint32_t i = tid; i < blockIdx.x; i += kThreads) ps += part[i];
+        uint32_t ptot;
+        block_excl_scan<kWaves>(ps, s_p, ptot);
+        base_sum = ptot;
+    } else {
+        base_sum = part[blockIdx.x];
+    }
     uint32_t s = 0;
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) s += x[i];
     uint32_t tot;
-    uint32_t run = block_excl_scan<kWaves>(s, s_w, tot) + part[blockIdx.x];
+    uint32_t run = block_excl_scan<kWaves>(s, s_w, tot) + base_sum;
 #pragma unroll
     for (int i = 0; i < kScanItems; ++i) {
         const uint32_t v = x[i];
@@ -127,10 +145,15 @@ hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t m, 
     hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kThreads), 0, stream, in, m, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (nb <= kScanFusedParts) {
+        hipLaunchKernelGGL((scan_down_kernel<true>), dim3(nb), dim3(kThreads), 0, stream, in, out, m,
+                           (const uint32_t*)part);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(1024), 0, stream, part, nb);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(scan_down_kernel, dim3(nb), dim3(kThreads), 0, stream, in, out, m,
+    hipLaunchKernelGGL((scan_down_kernel<false>), dim3(nb), dim3(kThreads), 0, stream, in, out, m,
                        (const uint32_t*)part);
     return hipGetLastError();
 }

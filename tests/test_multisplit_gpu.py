@@ -220,7 +220,7 @@ def test_prescan_matrix_stage(dev):
         gms.multisplit_release_memory(ctx)
 
 
-@pytest.mark.parametrize("m", [1, 5, 4095, 4096, 4097, 1 << 20, (1 << 22) + 3])
+@pytest.mark.parametrize("m", [1, 5, 4095, 4096, 4097, 1 << 20, (1 << 22) + 3, 2048 * 4096, 2048 * 4096 + 1])
 def test_exclusive_scan_stage(dev, m):
     import torch
     ctx = gms.multisplit_context(256)
