@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kThreads) void ms_batch_postscan_kernel(
     static_assert(NLEV >= 1 && NLEV <= 3, "at most three ranking passes");
     static_assert(K % 4 == 0, "vector loads");
 
-    constexpr uint32_t CNT = (1u << LEVEL_BITS) * kThreads;
+    constexpr uint32_t CNT = cnt_words(LEVEL_BITS);
     constexpr bool ALIAS = tune_alias(LOG_B, K, KV, LEVEL_BITS);
     __shared__ __attribute__((aligned(16))) uint32_t s_keys[TILE];
     __shared__ __attribute__((aligned(16))) uint32_t s_vc[KV ? TILE : CNT];

@@ -189,11 +189,14 @@ __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32
                                           uint32_t* s_vals, uint32_t* s_cnt, uint32_t* s_ws) {
     constexpr int D = 1 << W;
     constexpr uint32_t DMASK = (uint32_t)D - 1u;
+    constexpr uint32_t PAD = (uint32_t)cnt_pad(D), ROW = (uint32_t)cnt_row(D);
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
     const uint32_t first = tid * (uint32_t)K;
+    // LDS word of logical counter w = d*256 + t (rows of D padded to ROW).
+    auto cidx = [](uint32_t w) { return w + PAD * (w / (uint32_t)D); };
 
 #pragma unroll
-    for (int d = 0; d < D; ++d) s_cnt[d * kThreads + tid] = 0;
+    for (int d = 0; d < D; ++d) s_cnt[cidx((uint32_t)d * kThreads + tid)] = 0;
 
     uint32_t lr[K], dg[K];
 #pragma unroll
@@ -202,14 +205,14 @@ __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32
         lr[k] = 0;
         if (FULL || first + (uint32_t)k < nvalid) {
             const uint32_t d = (f(key[k]) >> SHIFT) & DMASK;
-            lr[k] = atomicAdd(&s_cnt[(d << 8) | tid], 1u);
+            lr[k] = atomicAdd(&s_cnt[cidx((d << 8) | tid)], 1u);
             dg[k] = d;
         }
     }
     __syncthreads();
 
     uint32_t c[D];
-    uint32_t* mine = s_cnt + tid * (uint32_t)D;
+    uint32_t* mine = s_cnt + tid * ROW;
     if constexpr (D % 4 == 0) {
 #pragma unroll
         for (int i = 0; i < D / 4; ++i) {
@@ -249,7 +252,7 @@ __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32
     if constexpr (ALIAS) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const uint32_t pos = s_cnt[(dg[k] << 8) | tid] + lr[k];
+            const uint32_t pos = s_cnt[cidx((dg[k] << 8) | tid)] + lr[k];
             lr[k] = pos < nvalid ? pos : nvalid - 1u;  // in bounds even if counts were inconsistent
         }
         __syncthreads();
@@ -264,7 +267,7 @@ __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (FULL || first + (uint32_t)k < nvalid) {
-                uint32_t pos = s_cnt[(dg[k] << 8) | tid] + lr[k];
+                uint32_t pos = s_cnt[cidx((dg[k] << 8) | tid)] + lr[k];
                 pos = pos < nvalid ? pos : nvalid - 1u;  // in bounds even if counts were inconsistent
                 s_keys[pos] = key[k];
                 if constexpr (KV) s_vals[pos] = val[k];
@@ -364,7 +367,7 @@ __global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
 
     // Key-value tiles keep the digit counters inside the value staging area
     // (ALIAS in rank_pass); key-only tiles have no values and a separate area.
-    constexpr uint32_t CNT = (1u << LEVEL_BITS) * kThreads;
+    constexpr uint32_t CNT = cnt_words(LEVEL_BITS);
     constexpr bool ALIAS = tune_alias(LOG_B, K, KV, LEVEL_BITS);
     __shared__ __attribute__((aligned(16))) uint32_t s_keys[TILE];
     __shared__ __attribute__((aligned(16))) uint32_t s_vc[KV ? TILE : CNT];

@@ -30,13 +30,24 @@ constexpr int tune_level_bits(int log_b) {
     return log_b <= 4 ? (log_b < 1 ? 1 : log_b) : (log_b <= 6 ? 3 : 4);
 }
 
+// Digit counters of a ranking pass with D digits: logically D x 256 u32,
+// digit-major (word d*256 + t), and each thread scans the D consecutive
+// logical words [t*D, t*D+D) -- its "row".  Rows are padded in LDS to
+// D + cnt_pad(D) words so that the rows of 8 consecutive lanes start on
+// different banks (row stride / 4 odd for 16-byte accesses; stride 3 for
+// D = 2): unpadded, the 16-byte row reads of D = 16 are 16-way bank conflicts.
+constexpr int cnt_pad(int D) { return D % 4 == 0 ? (((D / 4) & 1) ? 0 : 4) : (D == 2 ? 1 : 0); }
+constexpr int cnt_row(int D) { return D + cnt_pad(D); }
+// LDS words of the counters of passes of up to level_bits bits.
+constexpr uint32_t cnt_words(int level_bits) { return (uint32_t)cnt_row(1 << level_bits) * kThreads; }
+
 // Whether a key-value postscan keeps its digit counters inside the value
 // staging area (fewer LDS bytes, more resident workgroups).  Not for 128/256
 // buckets with short tiles: three resident workgroups per CU then keep more
 // partially written output lines open than L2 merges well, which measured
 // 35 % slower than two.
 constexpr bool tune_alias(int log_b, int items, bool kv, int level_bits) {
-    return kv && (1 << level_bits) <= items && !(log_b >= 7 && items < 28);
+    return kv && cnt_row(1 << level_bits) <= items && !(log_b >= 7 && items < 28);
 }
 
 // Keys per tile for an input of n keys.  The shortest tile, used to size the
