@@ -368,3 +368,80 @@ def test_2e25_kv_256_against_oracle(dev):
     keys = W.uniform_keys(n, seed=25)
     got = run_ms(dev, keys, B, gms.range_bucket(B), True)
     check_equal(got, oracle_ms(keys, B, oracle.RANGE), True)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("B,kv", [(256, False), (16, True)])
+def test_past_2e31_keys(dev, B, kv):
+    """More than 2^31 keys (8.6 GB per array): positions past the signed
+    32-bit range.  The GPU runs only the multisplit (inputs come from torch's
+    generators); results are checked on the host with numpy, chunk by chunk:
+    bucket order, offsets = scan of counts, the multiset of keys per bucket
+    (key-only), and for key-value (value = index) a permutation whose keys
+    travel with their values, stable inside each bucket."""
+    import torch
+    n = (1 << 31) + 12345
+    banner(f"past 2^31: n={n} B={B} kv={kv}")
+    g = torch.Generator(device=dev)
+    g.manual_seed(31)
+    kin = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=dev, generator=g)
+    kout = torch.empty_like(kin)
+    vin = torch.arange(n, dtype=torch.int64, device=dev).to(torch.int32) if kv else None
+    vout = torch.empty_like(kin) if kv else None
+    d_off = torch.zeros(B, dtype=torch.int32, device=dev)
+    ctx = gms.multisplit_context(B)
+    gms.multisplit_allocate_key_value(n, ctx)
+    gms.multisplit(kin, kout, n, ctx, gms.range_bucket(B), vin, vout, d_off)
+    torch.cuda.synchronize()
+    gms.multisplit_release_memory(ctx)
+    hk = from_dev(kin)
+    del kin, vin
+    ho = from_dev(kout)
+    del kout
+    hv = from_dev(vout) if kv else None
+    del vout
+    offs = from_dev(d_off)
+    CH = 1 << 26
+
+    def bid(a):
+        return ((a.astype(np.uint64) * np.uint64(B)) >> np.uint64(32)).astype(np.int64)
+
+    def key_sums(b, k):
+        # per-bucket sums of the low and high 16 bits (exact in float64)
+        return np.stack([np.bincount(b, weights=(k & np.uint32(0xFFFF)).astype(np.float64), minlength=B),
+                         np.bincount(b, weights=(k >> np.uint32(16)).astype(np.float64), minlength=B)])
+
+    cnt = np.zeros(B, np.int64)
+    ksum_in = np.zeros((2, B))
+    for c in range(0, n, CH):
+        b = bid(hk[c:c + CH])
+        cnt += np.bincount(b, minlength=B)
+        if not kv:
+            ksum_in += key_sums(b, hk[c:c + CH])
+    want_off = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    np.testing.assert_array_equal(offs.astype(np.int64), want_off)
+    prev_b = -1
+    ksum_out = np.zeros((2, B))
+    for c in range(0, n, CH):
+        b = bid(ho[c:c + CH])
+        assert b[0] >= prev_b and bool((b[1:] >= b[:-1]).all()), f"bucket order broken near {c}"
+        prev_b = int(b[-1])
+        if not kv:
+            ksum_out += key_sums(b, ho[c:c + CH])
+    if not kv:
+        np.testing.assert_array_equal(ksum_out, ksum_in)       # same keys in every bucket
+        return
+    seen = np.zeros(n, dtype=bool)
+    prev_v, prev_b = -1, -1
+    for c in range(0, n, CH):
+        v = hv[c:c + CH].astype(np.int64)
+        assert bool(((v >= 0) & (v < n)).all()), "output values outside [0, n)"
+        seen[v] = True
+        np.testing.assert_array_equal(hk[v], ho[c:c + CH])     # keys travel with values
+        b = bid(ho[c:c + CH])
+        bb = np.concatenate([[prev_b], b])
+        vv = np.concatenate([[prev_v], v])
+        same = bb[1:] == bb[:-1]
+        assert bool((vv[1:][same] > vv[:-1][same]).all()), f"unstable near {c}"
+        prev_v, prev_b = int(v[-1]), int(b[-1])
+    assert bool(seen.all())                                     # a permutation of 0..n-1
