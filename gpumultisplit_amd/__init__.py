@@ -279,6 +279,18 @@ def _digit_functor(bits: int):
     return _Digit
 
 
+class digit_bucket(_Bucket):
+    """(key >> shift) & (2^width - 1) for any width up to 20 bits (the
+    ms_bucket_sort_Nbit functors fix width to 2..8)."""
+    kind = BUCKET_DIGIT
+
+    def __init__(self, shift: int, width: int):
+        self.param, self.width = int(shift), int(width)
+
+    def __call__(self, a: int) -> int:
+        return (int(a) >> self.param) & ((1 << self.width) - 1)
+
+
 ms_bucket_sort_2bit = _digit_functor(2)
 ms_bucket_sort_4bit = _digit_functor(4)
 ms_bucket_sort_5bit = _digit_functor(5)
@@ -344,8 +356,10 @@ def _ms_allocate(num_elements: int, context: multisplit_context, kv: int) -> Non
            "multisplit_allocate")
     context.capacity = int(num_elements)
     context.size_sub_prob = int(lib().gms_subproblem_size(context.num_buckets, kv))
+    # no tile concept above 256 buckets (reduced-bit-sort path): 0 subproblems
     context.num_sub_prob = context.num_blocks = (
-        (int(num_elements) + context.size_sub_prob - 1) // context.size_sub_prob)
+        (int(num_elements) + context.size_sub_prob - 1) // context.size_sub_prob
+        if context.size_sub_prob else 0)
 
 
 def multisplit_allocate_key_only(num_elements: int, context: multisplit_context) -> None:

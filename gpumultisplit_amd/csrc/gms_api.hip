@@ -15,6 +15,7 @@ struct gms_multisplit_ctx {
     uint32_t capacity = 0;
     int allocated = 0;
     Scratch s;
+    LargeScratch large;  // more than 256 buckets only
     Profiler prof;
 };
 
@@ -130,7 +131,7 @@ gms_status check_desc(uint32_t num_buckets, gms_bucket_desc d) {
         case GMS_BUCKET_RANGE: return GMS_SUCCESS;
         case GMS_BUCKET_DELTA: return d.param >= 1 ? GMS_SUCCESS : GMS_ERR_INVALID_ARG;
         case GMS_BUCKET_DIGIT:
-            if (d.param >= 32 || d.width > 8) return GMS_ERR_INVALID_ARG;
+            if (d.param >= 32 || d.width > 20) return GMS_ERR_INVALID_ARG;
             if (d.width == 0) return is_pow2(num_buckets) ? GMS_SUCCESS : GMS_ERR_INVALID_ARG;
             return (1u << d.width) <= num_buckets ? GMS_SUCCESS : GMS_ERR_INVALID_ARG;
         case GMS_BUCKET_IDENTITY: return GMS_SUCCESS;
@@ -172,9 +173,12 @@ gms_status run_prescan_only(Scratch& s, const uint32_t* kin, uint32_t n, uint32_
     return hip_status(with_functor(B, d, [&](auto f) { return launch_prescan(c, f); }));
 }
 
-gms_status run_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
-                          uint32_t* vout, uint32_t n, uint32_t B, gms_bucket_desc d,
-                          uint32_t* d_offsets, hipStream_t stream, Profiler* prof) {
+// prescan -> scan -> postscan with the identifier that `with_f` hands to its
+// callback (the built-in kinds by descriptor, or a given functor).
+template <class WithF>
+static gms_status run_stages(Scratch& s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                             uint32_t* vout, uint32_t n, uint32_t B, uint32_t* d_offsets,
+                             hipStream_t stream, Profiler* prof, WithF&& with_f) {
     const bool kv = vin != nullptr;
     if (n == 0) return d_offsets ? hip_status(launch_fill(d_offsets, 0u, B, stream)) : GMS_SUCCESS;
     if (B == 1) {
@@ -194,14 +198,28 @@ gms_status run_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin, 
     hipEvent_t* ev = (prof && prof->on) ? prof->next(stream) : nullptr;
     hipError_t e = hipSuccess;
     if (ev) e = hipEventRecord(ev[0], stream);
-    if (e == hipSuccess) e = with_functor(B, d, [&](auto f) { return launch_prescan(c, f); });
+    if (e == hipSuccess) e = with_f([&](auto f) { return launch_prescan(c, f); });
     if (e == hipSuccess && ev) e = hipEventRecord(ev[1], stream);
     if (e == hipSuccess) e = launch_exclusive_scan(s.hist, s.hist, (uint32_t)m, s.part, stream);
     if (e == hipSuccess && ev) e = hipEventRecord(ev[2], stream);
-    if (e == hipSuccess) e = with_functor(B, d, [&](auto f) { return launch_postscan(c, f); });
+    if (e == hipSuccess) e = with_f([&](auto f) { return launch_postscan(c, f); });
     if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
     if (e == hipSuccess && d_offsets) e = launch_gather_offsets(s.hist, nt, B, d_offsets, stream);
     return hip_status(e);
+}
+
+gms_status run_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                          uint32_t* vout, uint32_t n, uint32_t B, gms_bucket_desc d,
+                          uint32_t* d_offsets, hipStream_t stream, Profiler* prof) {
+    return run_stages(s, kin, vin, kout, vout, n, B, d_offsets, stream, prof,
+                      [&](auto&& launch) { return with_functor(B, d, launch); });
+}
+
+gms_status run_multisplit_digit(Scratch& s, const uint32_t* kin, const uint32_t* vin,
+                                uint32_t* kout, uint32_t* vout, uint32_t n, uint32_t B,
+                                const DigitOfF& f, hipStream_t stream) {
+    return run_stages(s, kin, vin, kout, vout, n, B, nullptr, stream, nullptr,
+                      [&](auto&& launch) { return launch(f); });
 }
 
 int sort_schedule(uint32_t digit_bits, SortPass* p) {
@@ -243,7 +261,7 @@ const char* gms_status_string(gms_status s) {
     switch (s) {
         case GMS_SUCCESS: return "success";
         case GMS_ERR_INVALID_ARG: return "invalid argument";
-        case GMS_ERR_UNSUPPORTED_BUCKETS: return "unsupported number of buckets (1..256)";
+        case GMS_ERR_UNSUPPORTED_BUCKETS: return "unsupported number of buckets";
         case GMS_ERR_CAPACITY: return "n exceeds the allocated capacity";
         case GMS_ERR_NOT_ALLOCATED: return "context not allocated";
         case GMS_ERR_ALLOC: return "device allocation failed";
@@ -258,7 +276,7 @@ uint32_t gms_abi_version(void) { return GMS_ABI_VERSION; }
 
 gms_status gms_bucket_ids_host(uint32_t num_buckets, gms_bucket_desc desc, const uint32_t* keys,
                                uint32_t* ids, size_t n) {
-    if (num_buckets < 1 || num_buckets > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    if (num_buckets < 1 || num_buckets > kMaxBuckets) return GMS_ERR_UNSUPPORTED_BUCKETS;
     gms_status st = check_desc(num_buckets, desc);
     if (st != GMS_SUCCESS) return st;
     if (n && (!keys || !ids)) return GMS_ERR_INVALID_ARG;
@@ -283,7 +301,7 @@ uint32_t gms_tile_keys(uint32_t num_buckets, int key_value, uint32_t n) {
 gms_status gms_multisplit_create(uint32_t num_buckets, gms_multisplit_ctx** out) {
     if (!out) return GMS_ERR_INVALID_ARG;
     *out = nullptr;
-    if (num_buckets < 1 || num_buckets > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    if (num_buckets < 1 || num_buckets > kMaxBuckets) return GMS_ERR_UNSUPPORTED_BUCKETS;
     gms_multisplit_ctx* c = new (std::nothrow) gms_multisplit_ctx();
     if (!c) return GMS_ERR_ALLOC;
     c->num_buckets = num_buckets;
@@ -294,8 +312,13 @@ gms_status gms_multisplit_create(uint32_t num_buckets, gms_multisplit_ctx** out)
 gms_status gms_multisplit_allocate(gms_multisplit_ctx* ctx, uint32_t n_capacity, int key_value) {
     (void)key_value;  // the matrix is sized for both tile shapes
     if (!ctx) return GMS_ERR_INVALID_ARG;
-    gms_status st = scratch_alloc(ctx->s, matrix_elems(n_capacity, ctx->num_buckets));
-    if (st != GMS_SUCCESS) return st;
+    const bool large = ctx->num_buckets > 256;
+    gms_status st = scratch_alloc(ctx->s, matrix_elems(n_capacity, large ? 256u : ctx->num_buckets));
+    if (st == GMS_SUCCESS && large) st = large_scratch_alloc(ctx->large, n_capacity, ctx->num_buckets);
+    if (st != GMS_SUCCESS) {
+        scratch_free(ctx->s);
+        return st;
+    }
     ctx->capacity = n_capacity;
     ctx->allocated = 1;
     return GMS_SUCCESS;
@@ -304,6 +327,7 @@ gms_status gms_multisplit_allocate(gms_multisplit_ctx* ctx, uint32_t n_capacity,
 gms_status gms_multisplit_release(gms_multisplit_ctx* ctx) {
     if (!ctx) return GMS_ERR_INVALID_ARG;
     scratch_free(ctx->s);
+    large_scratch_free(ctx->large);
     ctx->prof.destroy();
     ctx->allocated = 0;
     ctx->capacity = 0;
@@ -337,13 +361,22 @@ static gms_status validate_call(gms_multisplit_ctx* ctx, const uint32_t* kin, co
     return GMS_SUCCESS;
 }
 
+static gms_status run_ctx(gms_multisplit_ctx* ctx, const uint32_t* kin, const uint32_t* vin,
+                          uint32_t* kout, uint32_t* vout, uint32_t n, gms_bucket_desc d,
+                          uint32_t* d_offsets, hipStream_t s) {
+    if (ctx->num_buckets > 256)
+        return run_large_multisplit(ctx->large, ctx->s, kin, vin, kout, vout, n, ctx->num_buckets,
+                                    d, d_offsets, s);
+    return run_multisplit(ctx->s, kin, vin, kout, vout, n, ctx->num_buckets, d, d_offsets, s,
+                          &ctx->prof);
+}
+
 gms_status gms_multisplit(gms_multisplit_ctx* ctx, const uint32_t* kin, const uint32_t* vin,
                           uint32_t* kout, uint32_t* vout, uint32_t n, gms_bucket_desc d,
                           uint32_t* d_offsets, void* stream) {
     gms_status st = validate_call(ctx, kin, vin, kout, vout, n, d);
     if (st != GMS_SUCCESS) return st;
-    return run_multisplit(ctx->s, kin, vin, kout, vout, n, ctx->num_buckets, d, d_offsets,
-                          (hipStream_t)stream, &ctx->prof);
+    return run_ctx(ctx, kin, vin, kout, vout, n, d, d_offsets, (hipStream_t)stream);
 }
 
 static gms_status multisplit_host_offsets(gms_multisplit_ctx* ctx, const uint32_t* kin,
@@ -353,10 +386,10 @@ static gms_status multisplit_host_offsets(gms_multisplit_ctx* ctx, const uint32_
     gms_status st = validate_call(ctx, kin, vin, kout, vout, n, d);
     if (st != GMS_SUCCESS) return st;
     hipStream_t s = (hipStream_t)stream;
-    st = run_multisplit(ctx->s, kin, vin, kout, vout, n, ctx->num_buckets, d,
-                        h_offsets ? ctx->s.offs : nullptr, s, &ctx->prof);
+    uint32_t* staging = ctx->num_buckets > 256 ? ctx->large.offs : ctx->s.offs;
+    st = run_ctx(ctx, kin, vin, kout, vout, n, d, h_offsets ? staging : nullptr, s);
     if (st != GMS_SUCCESS || !h_offsets) return st;
-    if (hipMemcpyAsync(h_offsets, ctx->s.offs, ctx->num_buckets * sizeof(uint32_t),
+    if (hipMemcpyAsync(h_offsets, staging, (size_t)ctx->num_buckets * sizeof(uint32_t),
                        hipMemcpyDeviceToHost, s) != hipSuccess)
         return GMS_ERR_HIP;
     return hip_status(hipStreamSynchronize(s));
@@ -401,6 +434,7 @@ gms_status gms_multisplit_prescan_matrix(gms_multisplit_ctx* ctx, const uint32_t
                                          void* stream) {
     if (!ctx) return GMS_ERR_INVALID_ARG;
     if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
+    if (ctx->num_buckets > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;  // no per-tile matrix there
     if (n > ctx->capacity) return GMS_ERR_CAPACITY;
     gms_status st = check_desc(ctx->num_buckets, d);
     if (st != GMS_SUCCESS) return st;

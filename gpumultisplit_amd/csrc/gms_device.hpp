@@ -124,6 +124,15 @@ struct AnyF {
     }
 };
 
+// One digit of another identifier's bucket id: (base(k) >> shift) & mask,
+// with base's clamp applied first.  The LSD passes of a multisplit into more
+// than 256 buckets rank by these digits of the (up to 20-bit) bucket id.
+struct DigitOfF {
+    AnyF base;
+    uint32_t shift, mask;
+    GMS_HD uint32_t operator()(uint32_t k) const { return (base(k) >> shift) & mask; }
+};
+
 // ---------------------------------------------------------------------------
 // Wave64 scans on DPP.  Rows are 16 lanes: four row_shr steps give prefixes
 // inside a row, row_bcast:15 / row_bcast:31 carry them across rows.

@@ -32,10 +32,12 @@ hipError_t launch_prescan(const MsCall& c, const RangeF& f);
 hipError_t launch_prescan(const MsCall& c, const DeltaF& f);
 hipError_t launch_prescan(const MsCall& c, const DigitF& f);
 hipError_t launch_prescan(const MsCall& c, const IdentF& f);
+hipError_t launch_prescan(const MsCall& c, const DigitOfF& f);
 hipError_t launch_postscan(const MsCall& c, const RangeF& f);
 hipError_t launch_postscan(const MsCall& c, const DeltaF& f);
 hipError_t launch_postscan(const MsCall& c, const DigitF& f);
 hipError_t launch_postscan(const MsCall& c, const IdentF& f);
+hipError_t launch_postscan(const MsCall& c, const DigitOfF& f);
 
 // Exclusive scan of m u32 (in may equal out); part holds scan_partials_needed(m).
 hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t m, uint32_t* part,
@@ -59,6 +61,23 @@ struct Scratch {
     uint32_t* offs = nullptr;  // 256 bucket starts (staging for host offsets)
     int device = -1;
 };
+
+// Multisplits into more than 256 buckets run as a stable radix sort of the
+// bucket ids (gms_large.hip); up to this many buckets.
+constexpr uint32_t kMaxBuckets = 1u << 20;
+
+struct LargeScratch {
+    uint32_t* ktmp = nullptr;  // keys between passes
+    uint32_t* vtmp = nullptr;  // values between passes
+    uint32_t* offs = nullptr;  // num_buckets bucket starts (staging for host offsets)
+    uint32_t cap = 0;
+};
+gms_status large_scratch_alloc(LargeScratch& L, uint32_t n_capacity, uint32_t num_buckets);
+void large_scratch_free(LargeScratch& L);
+gms_status run_large_multisplit(LargeScratch& L, Scratch& s, const uint32_t* kin,
+                                const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint32_t n,
+                                uint32_t num_buckets, gms_bucket_desc d, uint32_t* d_offsets,
+                                hipStream_t stream);
 
 gms_status scratch_alloc(Scratch& s, size_t hist_elems);
 void scratch_free(Scratch& s);
@@ -84,6 +103,12 @@ gms_status check_desc(uint32_t num_buckets, gms_bucket_desc d);
 gms_status run_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
                           uint32_t* vout, uint32_t n, uint32_t num_buckets, gms_bucket_desc d,
                           uint32_t* d_offsets, hipStream_t stream, Profiler* prof);
+
+// The same with one digit of a wide bucket id as the identifier (B = 2^width
+// of the digit, at most 256).
+gms_status run_multisplit_digit(Scratch& s, const uint32_t* kin, const uint32_t* vin,
+                                uint32_t* kout, uint32_t* vout, uint32_t n, uint32_t num_buckets,
+                                const DigitOfF& f, hipStream_t stream);
 
 // Prescan only (for the stage-level check).
 gms_status run_prescan_only(Scratch& s, const uint32_t* kin, uint32_t n, uint32_t num_buckets,

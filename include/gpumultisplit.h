@@ -26,7 +26,7 @@ extern "C" {
 typedef enum gms_status {
     GMS_SUCCESS = 0,
     GMS_ERR_INVALID_ARG = 1,        /* null/aliased pointer, bad descriptor     */
-    GMS_ERR_UNSUPPORTED_BUCKETS = 2,/* bucket count outside [1, 256]            */
+    GMS_ERR_UNSUPPORTED_BUCKETS = 2,/* bucket count outside the supported range */
     GMS_ERR_CAPACITY = 3,           /* n larger than the allocated capacity     */
     GMS_ERR_NOT_ALLOCATED = 4,      /* call before *_allocate                   */
     GMS_ERR_ALLOC = 5,              /* device allocation failed                 */
@@ -44,7 +44,8 @@ typedef enum gms_bucket_kind {
     GMS_BUCKET_DELTA = 1,     /* key / param           (delta_func,
                                  src/main/main_multisplit.cu:41-47)           */
     GMS_BUCKET_DIGIT = 2,     /* (key >> param) & (2^width - 1); width 0 means
-                                 log2(B) for a power-of-two B; 2^width <= B
+                                 log2(B) for a power-of-two B; 2^width <= B,
+                                 width <= 20
                                  (ms_bucket_sort_Nbit<x>,
                                  src/api/multisplit_sort.cuh:63-98)           */
     GMS_BUCKET_IDENTITY = 3,  /* key                   (identity_bucket,
@@ -77,7 +78,11 @@ gms_status gms_bucket_ids_host(uint32_t num_buckets, gms_bucket_desc desc,
 typedef struct gms_multisplit_ctx gms_multisplit_ctx;
 
 /* multisplit_context(uint32_t n_buckets) (src/api/multisplit.cuh:29-47).
- * 1 <= num_buckets <= 256 (the reference accepts powers of two 2..256). */
+ * 1 <= num_buckets <= 2^20.  Up to 256 buckets the multisplit runs as
+ * prescan -> scan -> postscan (the reference's API accepts powers of two
+ * 2..256); above 256 as a stable radix sort of the bucket ids, the reduced-bit
+ * sort the reference points to for more buckets (src/kernels/RBsort/
+ * reduced_bit_sort.cuh:24-57, src/main/main_wms.cu:534-755). */
 gms_status gms_multisplit_create(uint32_t num_buckets, gms_multisplit_ctx **out);
 
 /* multisplit_allocate_key_only(n, ctx)  (src/api/multisplit.cuh:50-61) with
@@ -137,7 +142,7 @@ gms_status gms_multisplit_profile(gms_multisplit_ctx *ctx, double *ms_prescan,
                                   double *ms_scan, double *ms_postscan,
                                   uint32_t *calls);
 
-/* Stage-level check: runs only the prescan on d_key_in and copies the per-tile
+/* Stage-level check (up to 256 buckets): runs only the prescan on d_key_in and copies the per-tile
  * bucket counts, bucket-major (H[b * tiles + t]), to d_matrix (B * tiles u32,
  * tiles = ceil(n / gms_tile_keys(B, key_value, n))).  Same quantity the reference's
  * cpu_prescan_test checks (src/cpu_functions.h:55-103). */

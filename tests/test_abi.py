@@ -102,7 +102,11 @@ def test_descriptor_validation_on_host():
         gms.bucket_ids_host(keys, 0, gms.range_bucket(0))
     assert e.value.status == 2
     with pytest.raises(gms.GmsError):
-        gms.bucket_ids_host(keys, 257, gms.range_bucket(257))
+        gms.bucket_ids_host(keys, (1 << 20) + 1, gms.range_bucket((1 << 20) + 1))
+    big = W.uniform_keys(1000, seed=257)                                # > 256 buckets: host check too
+    for B in (257, 4096, 1 << 20):
+        np.testing.assert_array_equal(gms.bucket_ids_host(big, B, gms.range_bucket(B)),
+                                      oracle.bucket_ids(big, B, oracle.RANGE))
     with pytest.raises(gms.GmsError):
         gms.bucket_ids_host(keys, 6, gms.ms_bucket_sort_4bit(0))      # 16 digit values > 6 buckets
     np.testing.assert_array_equal(                                    # 4 digit values fit 6 buckets
@@ -128,6 +132,16 @@ def test_context_creation_checks_bucket_count():
         gms.multisplit_context(0)
     assert e.value.status == 2
     with pytest.raises(gms.GmsError):
-        gms.multisplit_context(300)
+        gms.multisplit_context((1 << 20) + 1)
+    assert gms.multisplit_context(300).num_buckets == 300             # reduced-bit sort path
     ctx = gms.multisplit_context(256)       # host-only object, no HIP call yet
     assert ctx.num_buckets == 256
+
+
+def test_library_has_no_undefined_library_symbols():
+    """Every gms:: symbol the library references is defined in it (a shared
+    library links even when one is missing; loading then fails)."""
+    r = subprocess.run(["nm", "-D", "-C", "--undefined-only", gms.LIB_PATH], capture_output=True,
+                       text=True, check=True)
+    missing = [ln for ln in r.stdout.splitlines() if "gms::" in ln or " gms_" in ln]
+    assert not missing, missing

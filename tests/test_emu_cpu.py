@@ -291,3 +291,34 @@ def test_emulated_server_budget_admission(emu):
         order = np.argsort(keys, kind="stable")
         np.testing.assert_array_equal(kout, keys[order])
         np.testing.assert_array_equal(vout, vals[order])
+
+
+@pytest.mark.parametrize("B,kind,param,width", [(257, oracle.RANGE, 0, 0), (1000, oracle.DELTA, 4294968, 0),
+                                                (4096, oracle.DIGIT, 7, 12), (70000, oracle.IDENTITY, 0, 0),
+                                                (1 << 20, oracle.RANGE, 0, 0)])
+def test_emulated_more_than_256_buckets(emu, B, kind, param, width):
+    """Reduced-bit-sort path: ids, LSD passes over the id bits, gather."""
+    n = 7001
+    keys = W.uniform_keys(n, seed=B)
+    if kind == oracle.IDENTITY:
+        keys = keys % np.uint32(B + 100)
+    desc = gms.gms_bucket_desc(kind, param, width)
+    for kv in (False, True):
+        vals = np.arange(n, dtype=np.uint32)
+        kout, vout = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+        offs = np.zeros(B, np.uint32)
+        h = ctypes.c_void_p()
+        assert emu.gms_multisplit_create(B, ctypes.byref(h)) == 0
+        assert emu.gms_multisplit_allocate(h, n, int(kv)) == 0
+        if kv:
+            st = emu.gms_multisplit_key_value(h, p(keys), p(vals), p(kout), p(vout), n, desc, p(offs), None)
+        else:
+            st = emu.gms_multisplit_key_only(h, p(keys), p(kout), n, desc, p(offs), None)
+        emu.gms_multisplit_destroy(h)
+        assert st == 0
+        okind = oracle.DIGIT if kind == oracle.DIGIT else kind
+        wk, wv, wo = oracle.multisplit(keys, B, okind, param, vals)
+        np.testing.assert_array_equal(kout, wk)
+        np.testing.assert_array_equal(offs, wo)
+        if kv:
+            np.testing.assert_array_equal(vout, wv)
