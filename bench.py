@@ -36,7 +36,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--workload", default="ms_kv_256",
                    help="ms_{kv,ko}_<B>[_<dist>][_n<log2n>] | sort{7,8}_{kv,ko} | "
-                        "hist256 | compact{50,5} | copy")
+                        "hist256 | histrange256 | compact{50,5} | copy")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-stage-events", action="store_true")
     p.add_argument("--cpu-reps", type=int, default=5)
@@ -211,6 +211,8 @@ def bench_aux(args, torch, gms, W, dev):
     """The other hot-path rows, one JSON line each (measurement runs):
     sort{7,8}_{kv,ko}: radix sort of 2^25 uint32 keys (value = index);
     hist256: 256-bin histogram of the low 8 bits of 2^27 keys;
+    histrange256: 256 custom bins (257 sorted random edges in [0, 1024)) of
+      2^27 float samples uniform in [0, 1024), as the reference's range driver;
     compact{50,5}: 2-bucket compaction of 2^27 keys, predicate key < p*2^32;
     copy: the streaming copy kernel over 2^27 u32 (the measured copy ceiling)."""
     import numpy as np
@@ -227,6 +229,8 @@ def bench_aux(args, torch, gms, W, dev):
         metric = f"{'key-value' if kv else 'key-only'} radix sort ({bits}-bit digits), {n} uint32 keys"
     elif w == "hist256":
         n, bpk, metric = 1 << 27, 4, "256-bin histogram of the low 8 bits, 134217728 uint32 keys"
+    elif w == "histrange256":
+        n, bpk, metric = 1 << 27, 4, "256-bin custom-edge histogram, 134217728 float samples"
     elif w.startswith("compact"):
         pct = int(w[7:])
         n, bpk = 1 << 27, 12
@@ -256,6 +260,17 @@ def bench_aux(args, torch, gms, W, dev):
             st = L.gms_histogram_u32(kin.data_ptr(), n, h.data_ptr(), 256, desc, stream)
             if st != 0:
                 raise gms.GmsError(st, "gms_histogram_u32")
+    elif w == "histrange256":
+        h = torch.zeros(256, dtype=torch.int32, device=dev)
+        kin = torch.rand(n, device=dev, generator=g) * 1024.0
+        lv = torch.sort(torch.rand(257, device=dev, generator=g) * 1024.0).values
+        lv[0], lv[-1] = 0.0, 1024.0
+
+        def step():
+            st = L.gms_histogram_range_f32(kin.data_ptr(), n, h.data_ptr(), 257, lv.data_ptr(),
+                                           stream)
+            if st != 0:
+                raise gms.GmsError(st, "gms_histogram_range_f32")
     elif w.startswith("compact"):
         ctx = gms.compaction_context()
         gms.compaction_allocate_key_only(n, ctx)

@@ -123,6 +123,8 @@ _SIGS = [
     ("gms_histogram_u32", _I, [_P, _U32, _P, _U32, gms_bucket_desc, _P]),
     ("gms_histogram_even_f32", _I, [_P, _U32, _P, _U32, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_float, _P]),
+    ("gms_histogram_range_f32", _I, [_P, _U32, _P, _U32, _P, _P]),
+    ("gms_histogram_range_u32", _I, [_P, _U32, _P, _U32, _P, _P]),
     ("gms_batch_create", _I, [_PP]),
     ("gms_batch_destroy", _I, [_P]),
     ("gms_batch_plan", _I, [_P, ctypes.POINTER(gms_batch_item), _U32, _P]),
@@ -612,6 +614,18 @@ def multisplit2_histogram_even(d_key_in, num_elements: int, d_histogram, lower_l
                                         int(num_bins), float(lower_level), float(upper_level),
                                         float(inverse_delta), _stream(d_key_in)),
            "histogram_even")
+
+
+def multisplit2_histogram_range(d_key_in, num_elements: int, d_histogram, d_levels,
+                                num_levels: int) -> None:
+    """Counts per custom bin [levels[i], levels[i+1]) added into d_histogram
+    (num_levels - 1 bins); float32 samples/levels, or 32-bit integer ones
+    read as uint32 (src/kernels/histogram/multisplit2_histograms.cuh:397-699)."""
+    import torch
+    is_float = hasattr(d_key_in, "dtype") and d_key_in.dtype == torch.float32
+    fn = lib().gms_histogram_range_f32 if is_float else lib().gms_histogram_range_u32
+    _check(fn(_ptr(d_key_in), int(num_elements), _ptr(d_histogram), int(num_levels),
+              _ptr(d_levels), _stream(d_key_in)), "histogram_range")
 
 
 def copy_u32(d_src, d_dst, n: int) -> None:

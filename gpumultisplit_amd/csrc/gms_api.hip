@@ -639,6 +639,26 @@ gms_status gms_histogram_even_f32(const float* x, uint32_t n, int32_t* d_hist, u
         launch_histogram_even_f32(x, n, d_hist, num_bins, lower, inv_delta, (hipStream_t)stream));
 }
 
+gms_status gms_histogram_range_f32(const float* d_samples, uint32_t n, int32_t* d_hist,
+                                   uint32_t num_levels, const float* d_levels, void* stream) {
+    if (num_levels < 2 || num_levels > 257) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    if (!d_hist || !d_levels) return GMS_ERR_INVALID_ARG;
+    if (n == 0) return GMS_SUCCESS;
+    if (!d_samples) return GMS_ERR_INVALID_ARG;
+    return hip_status(launch_histogram_range_f32(d_samples, n, d_hist, num_levels, d_levels,
+                                                 (hipStream_t)stream));
+}
+
+gms_status gms_histogram_range_u32(const uint32_t* d_keys, uint32_t n, int32_t* d_hist,
+                                   uint32_t num_levels, const uint32_t* d_levels, void* stream) {
+    if (num_levels < 2 || num_levels > 257) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    if (!d_hist || !d_levels) return GMS_ERR_INVALID_ARG;
+    if (n == 0) return GMS_SUCCESS;
+    if (!d_keys) return GMS_ERR_INVALID_ARG;
+    return hip_status(launch_histogram_range_u32(d_keys, n, d_hist, num_levels, d_levels,
+                                                 (hipStream_t)stream));
+}
+
 // ------------------------------------------------------------------- utility
 gms_status gms_copy_u32(const uint32_t* src, uint32_t* dst, size_t n, void* stream) {
     if (n == 0) return GMS_SUCCESS;

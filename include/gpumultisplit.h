@@ -230,6 +230,20 @@ gms_status gms_histogram_even_f32(const float *d_samples, uint32_t n, int32_t *d
                                   uint32_t num_bins, float lower, float upper,
                                   float inv_delta, void *stream);
 
+/* Histogram with custom bin edges: levels (device, num_levels ascending
+ * values, 2 <= num_levels <= 257) define num_levels - 1 bins [levels[i],
+ * levels[i+1]); samples outside [levels[0], levels[num_levels-1]) and NaN are
+ * not counted.  Adds into d_hist[num_levels - 1].  Replaces
+ * multisplit2_histogram_range_{128,64bin_128,128bin_128,256bin_128}(KeyT* kin,
+ * int n, int* d_hist, KeyT* d_levels) (src/kernels/histogram/
+ * multisplit2_histograms.cuh:397-699), with the bin semantics of CUB's
+ * HistogramRange that the reference's driver compares them with
+ * (src/main/main_histogram.cu:224-310). */
+gms_status gms_histogram_range_f32(const float *d_samples, uint32_t n, int32_t *d_hist,
+                                   uint32_t num_levels, const float *d_levels, void *stream);
+gms_status gms_histogram_range_u32(const uint32_t *d_keys, uint32_t n, int32_t *d_hist,
+                                   uint32_t num_levels, const uint32_t *d_levels, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Batched multisplit (no counterpart in the reference)                      */
 /* ------------------------------------------------------------------------ */

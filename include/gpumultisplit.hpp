@@ -337,6 +337,28 @@ inline void multisplit2_histogram_even(const float* d_key_in, uint32_t num_eleme
                "multisplit2_histogram_even");
 }
 
+// multisplit2_histogram_range_* (src/kernels/histogram/multisplit2_histograms.cuh:397-699):
+// num_levels ascending edges, bin i = [levels[i], levels[i+1]), samples outside
+// [levels[0], levels[num_levels-1]) and NaN not counted; float or 32-bit integer
+// samples (integers compared as uint32).
+inline void multisplit2_histogram_range(const float* d_key_in, uint32_t num_elements, int* d_hist,
+                                        const float* d_levels, uint32_t num_levels,
+                                        void* stream = nullptr) {
+    gms::check(gms_histogram_range_f32(d_key_in, num_elements, reinterpret_cast<int32_t*>(d_hist),
+                                       num_levels, d_levels, stream),
+               "multisplit2_histogram_range");
+}
+template <typename KeyT>
+inline void multisplit2_histogram_range(const KeyT* d_key_in, uint32_t num_elements, int* d_hist,
+                                        const KeyT* d_levels, uint32_t num_levels,
+                                        void* stream = nullptr) {
+    gms::require_32bit<KeyT>("key");
+    gms::check(gms_histogram_range_u32(reinterpret_cast<const uint32_t*>(d_key_in), num_elements,
+                                       reinterpret_cast<int32_t*>(d_hist), num_levels,
+                                       reinterpret_cast<const uint32_t*>(d_levels), stream),
+               "multisplit2_histogram_range");
+}
+
 // Integer-bin histogram through any bucket functor with a gms_bucket_desc
 // (the 64/128/256-bin kernels of multisplit2_histograms.cuh take the bin as a
 // bucket id of the key the same way).

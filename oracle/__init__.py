@@ -58,6 +58,10 @@ def lib():
         L.ms_oracle_histogram_even_f32.argtypes = [u32p, ctypes.c_uint64, u32p, ctypes.c_uint32,
                                                    ctypes.c_float, ctypes.c_float]
         L.ms_oracle_histogram_even_f32.restype = None
+        for name in ("ms_oracle_histogram_range_f32", "ms_oracle_histogram_range_u32"):
+            fn = getattr(L, name)
+            fn.argtypes = [u32p, ctypes.c_uint64, u32p, u32p, ctypes.c_uint32]
+            fn.restype = None
         L.ms_oracle_compaction.argtypes = [ctypes.c_uint32] * 2 + [u32p] * 4 + [ctypes.c_uint64]
         L.ms_oracle_compaction.restype = ctypes.c_uint64
         L.ms_oracle_tile_histograms.argtypes = [ctypes.c_uint32] * 3 + [u32p, ctypes.c_uint64,
@@ -126,6 +130,21 @@ def histogram_even_f32(samples, num_bins: int, lower: float, inv_delta: float) -
     h = np.zeros(num_bins, dtype=np.int32)
     lib().ms_oracle_histogram_even_f32(_ptr(x), x.size, _ptr(h), num_bins,
                                        ctypes.c_float(lower), ctypes.c_float(inv_delta))
+    return h
+
+
+def histogram_range(samples, levels) -> np.ndarray:
+    """Counts per bin [levels[i], levels[i+1]) of float32 or uint32 samples."""
+    if np.asarray(samples).dtype == np.float32:
+        x = np.ascontiguousarray(samples, dtype=np.float32)
+        lv = np.ascontiguousarray(levels, dtype=np.float32)
+        fn = lib().ms_oracle_histogram_range_f32
+    else:
+        x = _u32(samples)
+        lv = _u32(levels)
+        fn = lib().ms_oracle_histogram_range_u32
+    h = np.zeros(max(lv.size - 1, 0), dtype=np.int32)
+    fn(_ptr(x), x.size, _ptr(h), _ptr(lv), lv.size)
     return h
 
 

@@ -22,6 +22,8 @@
  *   ms_oracle_sort           numpy sort / stable argsort of the keys.
  *   ms_oracle_histogram      numpy bincount.
  *   ms_oracle_histogram_even numpy float32 arithmetic + bincount.
+ *   ms_oracle_histogram_range numpy.searchsorted(side="right") - 1 + bincount,
+ *                            incl. samples on the edges, outside, and NaN.
  *   ms_oracle_compaction     Python list partition; count = partition point.
  * The reference itself ships no golden vectors or known-answer tests; the
  * fixtures under tests/golden/ are produced by this oracle and cross-checked by
@@ -214,6 +216,41 @@ void ms_oracle_histogram_even_f32(const float *x, uint64_t n, int32_t *hist,
         if (!(s < (float)nbins)) continue;
         uint32_t b = (uint32_t)s;
         if (b < nbins) hist[b]++;
+    }
+}
+
+/*
+ * Histogram with custom bin edges: levels[0..nl) ascending, bin i counts the
+ * samples with levels[i] <= x < levels[i+1]; samples below levels[0], at or
+ * above levels[nl-1], and NaN are ignored -- the semantics of CUB's
+ * HistogramRange, which the reference's range histograms are compared with
+ * (src/main/main_histogram.cu:224-310; kernels multisplit2_histograms.cuh:397-699,
+ * bin = UpperBound(levels, x) - 1).  Linear scan of the levels here; the tests
+ * pin it against numpy.searchsorted.  Accumulates into hist (nl - 1 bins).
+ */
+void ms_oracle_histogram_range_f32(const float *x, uint64_t n, int32_t *hist,
+                                   const float *levels, uint32_t nl)
+{
+    if (nl < 2) return;
+    for (uint64_t i = 0; i < n; ++i) {
+        const float v = x[i];
+        if (!(v >= levels[0] && v < levels[nl - 1])) continue;   /* out of range or NaN */
+        uint32_t b = 0;
+        while (b + 2 < nl && !(v < levels[b + 1])) ++b;
+        hist[b]++;
+    }
+}
+
+void ms_oracle_histogram_range_u32(const uint32_t *x, uint64_t n, int32_t *hist,
+                                   const uint32_t *levels, uint32_t nl)
+{
+    if (nl < 2) return;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t v = x[i];
+        if (!(v >= levels[0] && v < levels[nl - 1])) continue;
+        uint32_t b = 0;
+        while (b + 2 < nl && !(v < levels[b + 1])) ++b;
+        hist[b]++;
     }
 }
 

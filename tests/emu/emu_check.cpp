@@ -21,6 +21,10 @@ void ms_oracle_histogram(uint32_t kind, uint32_t B, uint32_t param, const uint32
                          uint64_t n, uint32_t* hist);
 uint64_t ms_oracle_compaction(uint32_t kind, uint32_t param, const uint32_t* kin,
                               const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint64_t n);
+void ms_oracle_histogram_range_f32(const float* x, uint64_t n, int32_t* hist, const float* levels,
+                                   uint32_t nl);
+void ms_oracle_histogram_range_u32(const uint32_t* x, uint64_t n, int32_t* hist,
+                                   const uint32_t* levels, uint32_t nl);
 void ms_oracle_tile_histograms(uint32_t kind, uint32_t B, uint32_t param, const uint32_t* keys,
                                uint64_t n, uint32_t tile, uint32_t* H);
 }
@@ -205,6 +209,31 @@ int main(int argc, char** argv) {
                "histogram status");
         ms_oracle_histogram(GMS_BUCKET_DIGIT, 256, 0, kin.p, n, wh.data());
         EXPECT(memcmp(hist.p, wh.data(), 256 * 4) == 0, "histogram n=%u", n);
+
+        // custom bin edges: 33 and 257 levels, integer and float samples
+        for (uint32_t nl : {2u, 33u, 257u}) {
+            Buf lv(nl), hr(nl - 1, 0);
+            for (uint32_t i = 0; i < nl; ++i) lv.p[i] = 1000u + i * (0xF0000000u / nl);
+            std::vector<int32_t> wr(nl - 1, 0);
+            EXPECT(gms_histogram_range_u32(kin.p, n, (int32_t*)hr.p, nl, lv.p, nullptr) ==
+                       GMS_SUCCESS,
+                   "histogram_range_u32 status");
+            ms_oracle_histogram_range_u32(kin.p, n, wr.data(), lv.p, nl);
+            EXPECT(memcmp(hr.p, wr.data(), (nl - 1) * 4) == 0, "histogram_range_u32 n=%u nl=%u", n,
+                   nl);
+            std::vector<float> xs(n ? n : 1), fl(nl);
+            for (uint32_t i = 0; i < n; ++i) xs[i] = (float)(kin.p[i] >> 8) * (1.0f / 65536.0f) - 8.0f;
+            if (n > 3) xs[3] = __builtin_nanf("");
+            for (uint32_t i = 0; i < nl; ++i) fl[i] = -6.0f + 240.0f * (float)i / (float)nl;
+            Buf hf(nl - 1, 0);
+            std::vector<int32_t> wf(nl - 1, 0);
+            EXPECT(gms_histogram_range_f32(xs.data(), n, (int32_t*)hf.p, nl, fl.data(), nullptr) ==
+                       GMS_SUCCESS,
+                   "histogram_range_f32 status");
+            ms_oracle_histogram_range_f32(xs.data(), n, wf.data(), fl.data(), nl);
+            EXPECT(memcmp(hf.p, wf.data(), (nl - 1) * 4) == 0, "histogram_range_f32 n=%u nl=%u", n,
+                   nl);
+        }
     }
     printf("compaction/histogram done\n");
 

@@ -59,6 +59,12 @@ void launch(dim3 grid, dim3 block, std::function<void()> body);
 
 inline uint32_t atomicAdd(uint32_t* p, uint32_t v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline uint32_t atomicMax(uint32_t* p, uint32_t v) {
+    uint32_t old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+    while (old < v && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+    }
+    return old;
+}
 inline uint32_t __umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
 inline float __uint_as_float(uint32_t u) {
     float f;
@@ -67,6 +73,14 @@ inline float __uint_as_float(uint32_t u) {
 }
 inline float __fmul_rn(float a, float b) {
     volatile float r = a * b;
+    return r;
+}
+inline float __fsub_rn(float a, float b) {
+    volatile float r = a - b;
+    return r;
+}
+inline float __fdiv_rn(float a, float b) {
+    volatile float r = a / b;
     return r;
 }
 

@@ -16,6 +16,7 @@ import pytest
 import gpumultisplit_amd as gms
 import oracle
 from gpumultisplit_amd import workloads as W
+from hist_cases import range_level_sets, range_samples
 
 EMU_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "emu")
 EMU_LIB = os.path.join(EMU_DIR, "libgpumultisplit_emu.so")
@@ -322,3 +323,25 @@ def test_emulated_more_than_256_buckets(emu, B, kind, param, width):
         np.testing.assert_array_equal(offs, wo)
         if kv:
             np.testing.assert_array_equal(vout, wv)
+
+
+def test_emulated_histogram_range_edge_sets(emu):
+    # Custom-edge histograms through the cell-table kernel: spread edges,
+    # 200 edges inside one cell, duplicate edges, a range whose width
+    # overflows to inf, ranges a few ulps wide and denormal, integer ranges
+    # of every width; samples include every edge, its neighbours, NaN, +-inf.
+    sets, isets = range_level_sets()
+    for lv in sets:
+        x = range_samples(lv, 3 * 4096 + 123, seed=5)
+        h = np.zeros(lv.size - 1, np.int32)
+        assert emu.gms_histogram_range_f32(p(x), x.size, p(h), lv.size, p(lv), None) == 0
+        want = oracle.histogram_range(x, lv)
+        assert want.sum() > lv.size // 2
+        np.testing.assert_array_equal(h, want, err_msg=str(lv[:8]))
+    for lv in isets:
+        x = range_samples(lv, 3 * 4096 + 123, seed=6)
+        h = np.zeros(lv.size - 1, np.int32)
+        assert emu.gms_histogram_range_u32(p(x), x.size, p(h), lv.size, p(lv), None) == 0
+        want = oracle.histogram_range(x, lv)
+        assert want.sum() > lv.size // 2
+        np.testing.assert_array_equal(h, want, err_msg=str(lv[:8]))

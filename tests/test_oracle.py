@@ -226,3 +226,29 @@ def test_checks_catch_a_planted_stability_bug():
     ref = keys[np.argsort(ids, kind="stable")]
     assert np.array_equal(good, ref)
     assert not np.array_equal(bad, ref)
+
+
+def _np_histogram_range(x, levels):
+    b = np.searchsorted(levels, x, side="right") - 1
+    ok = (b >= 0) & (b < levels.size - 1)
+    if x.dtype == np.float32:
+        ok &= ~np.isnan(x)
+    return np.bincount(b[ok], minlength=levels.size - 1).astype(np.int32)
+
+
+@pytest.mark.parametrize("nbins", [1, 2, 7, 32, 100, 256])
+def test_oracle_histogram_range_against_searchsorted(nbins):
+    g = W.rng(nbins)
+    # float: random sorted edges in [-10, 300), samples around them incl. edges, outside, NaN
+    lv = np.sort(g.uniform(-10, 300, size=nbins + 1).astype(np.float32))
+    x = g.uniform(-20, 320, size=20000).astype(np.float32)
+    x[:nbins + 1] = lv                               # exactly on every edge
+    x[-3:] = np.array([np.nan, lv[0] - 1, lv[-1]], np.float32)
+    np.testing.assert_array_equal(oracle.histogram_range(x, lv), _np_histogram_range(x, lv))
+    # uint32, with repeated edges (empty bins)
+    lu = np.sort(g.integers(0, 1 << 32, size=nbins + 1, dtype=np.uint64)).astype(np.uint32)
+    if nbins > 3:
+        lu[2] = lu[1]
+    k = W.uniform_keys(20000, seed=nbins + 1)
+    k[:nbins + 1] = lu
+    np.testing.assert_array_equal(oracle.histogram_range(k, lu), _np_histogram_range(k, lu))

@@ -5,7 +5,8 @@ Sort: the reference checks its multisplit sort against CUB's radix sort
 CPU oracle (LSD over the same digit schedule) and numpy's stable sort, with
 values = original index so that instability shows.  Histogram: the
 reference compares against CUB's HistogramEven (src/main/main_histogram.cu:
-185-186); here numpy bincount / the oracle's float formula.  Compaction: the
+185-186) and CUB's HistogramRange (main_histogram.cu:224-310); here numpy
+bincount / the oracle's float formula / the oracle's edge scan.  Compaction: the
 reference compares with its CPU multisplit (src/main/main_compaction.cu:
 102-115, 167-181); here the oracle's stable partition and its count.
 """
@@ -16,6 +17,7 @@ import gpumultisplit_amd as gms
 import oracle
 from gpumultisplit_amd import workloads as W
 from gpu_util import banner, filled_u32, from_dev, to_dev
+from hist_cases import range_level_sets, range_samples
 
 pytestmark = pytest.mark.gpu
 
@@ -148,6 +150,70 @@ def test_histogram_even_float_like_reference_driver(dev):
     gms.multisplit2_histogram_even(torch.from_numpy(x).to(dev), n, h, lower, upper, inv, bins)
     np.testing.assert_array_equal(from_dev(h).view(np.int32),
                                   oracle.histogram_even_f32(x, bins, lower, inv))
+
+
+@pytest.mark.parametrize("num_levels", [2, 3, 33, 65, 129, 257])
+def test_histogram_range_float(dev, num_levels):
+    import torch
+    # main_histogram.cu:224-310 drives the range histograms with float samples
+    # and edges; here uneven ascending edges over [-10, 300), samples spilling
+    # out of range on both sides, samples exactly on edges, and NaN.
+    g = W.rng(num_levels)
+    levels = np.sort(g.random(num_levels, dtype=np.float32) * np.float32(310) - np.float32(10))
+    levels[0], levels[-1] = np.float32(-10), np.float32(300)
+    for n, off in ((0, 0), (1, 0), (4095, 1), (4096 * 3 + 7, 0), (1_000_003, 2)):
+        banner(f"range f32 nl={num_levels} n={n} off={off}")
+        x = (g.random(n, dtype=np.float32) * np.float32(340) - np.float32(25)).astype(np.float32)
+        if n > 10:
+            x[:num_levels if num_levels < n else n] = levels[:n]
+            x[-1], x[-2] = np.float32("nan"), np.float32(300)
+        xb = torch.from_numpy(np.concatenate([np.zeros(off, np.float32), x])).to(dev)
+        h = torch.zeros(num_levels - 1, dtype=torch.int32, device=dev)
+        gms.multisplit2_histogram_range(xb[off:], n, h, torch.from_numpy(levels).to(dev), num_levels)
+        np.testing.assert_array_equal(from_dev(h).view(np.int32), oracle.histogram_range(x, levels))
+
+
+@pytest.mark.parametrize("num_levels", [2, 17, 256, 257])
+def test_histogram_range_uint(dev, num_levels):
+    import torch
+    g = W.rng(1000 + num_levels)
+    levels = np.unique(g.integers(0, 1 << 32, num_levels * 2, dtype=np.uint64)).astype(np.uint32)
+    levels = np.sort(levels[:num_levels])
+    if num_levels == 257:
+        levels[0], levels[-1] = 0, 0xFFFFFFFF
+    for n in (0, 5, 123_457, (1 << 22) + 3):
+        banner(f"range u32 nl={num_levels} n={n}")
+        keys = W.uniform_keys(n, seed=n + num_levels)
+        h = torch.zeros(num_levels - 1, dtype=torch.int32, device=dev)
+        gms.multisplit2_histogram_range(dev_in(keys, dev), n, h, to_dev(levels, dev), num_levels)
+        np.testing.assert_array_equal(from_dev(h).view(np.int32), oracle.histogram_range(keys, levels))
+
+
+@pytest.mark.parametrize("case", range(13))
+def test_histogram_range_edge_sets(dev, case):
+    import torch
+    sets, isets = range_level_sets()
+    lv = (sets + isets)[case]
+    for n_rand in (3 * 4096 + 123, (1 << 22) + 1):
+        x = range_samples(lv, n_rand, seed=case)
+        banner(f"range edge set {case} ({lv.dtype}, {lv.size} levels) n={x.size}")
+        h = torch.zeros(lv.size - 1, dtype=torch.int32, device=dev)
+        if lv.dtype == np.float32:
+            gms.multisplit2_histogram_range(torch.from_numpy(x).to(dev), x.size, h,
+                                            torch.from_numpy(lv).to(dev), lv.size)
+        else:
+            gms.multisplit2_histogram_range(to_dev(x, dev), x.size, h, to_dev(lv, dev), lv.size)
+        np.testing.assert_array_equal(from_dev(h).view(np.int32), oracle.histogram_range(x, lv))
+
+
+def test_histogram_range_rejects_bad_levels(dev):
+    import torch
+    x = torch.zeros(16, dtype=torch.float32, device=dev)
+    h = torch.zeros(300, dtype=torch.int32, device=dev)
+    lv = torch.zeros(300, dtype=torch.float32, device=dev)
+    for nl in (0, 1, 258):
+        with pytest.raises(gms.GmsError):
+            gms.multisplit2_histogram_range(x, 16, h, lv, nl)
 
 
 @pytest.mark.parametrize("selectivity", [0.5, 0.05])
