@@ -36,7 +36,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--workload", default="ms_kv_256",
                    help="ms_{kv,ko}_<B>[_<dist>][_n<log2n>] | sort{7,8}_{kv,ko} | "
-                        "hist256 | histrange256 | compact{50,5} | copy")
+                        "hist256 | histrange256 | compact{50,5} | copy | wide_{kv,ko}_256 | wide_sort8_kv")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-stage-events", action="store_true")
     p.add_argument("--cpu-reps", type=int, default=5)
@@ -214,7 +214,11 @@ def bench_aux(args, torch, gms, W, dev):
     histrange256: 256 custom bins (257 sorted random edges in [0, 1024)) of
       2^27 float samples uniform in [0, 1024), as the reference's range driver;
     compact{50,5}: 2-bucket compaction of 2^27 keys, predicate key < p*2^32;
-    copy: the streaming copy kernel over 2^27 u32 (the measured copy ceiling)."""
+    copy: the streaming copy kernel over 2^27 u32 (the measured copy ceiling);
+    wide_kv_256 / wide_ko_256: 256-bucket range multisplit of 2^27 uint64 keys
+      with (or without) uint64 values;
+    wide_sort8_kv: 8-bit-digit radix sort of 2^25 uint64 keys with uint64
+      values (8 passes)."""
     import numpy as np
     w = args.workload
     stream = torch.cuda.current_stream(dev).cuda_stream
@@ -237,9 +241,20 @@ def bench_aux(args, torch, gms, W, dev):
         metric = f"2-bucket compaction ({pct}% kept), 134217728 uint32 keys"
     elif w == "copy":
         n, bpk, metric = 1 << 27, 8, "device copy of 134217728 uint32"
+    elif w in ("wide_kv_256", "wide_ko_256"):
+        kv = w == "wide_kv_256"
+        n, bpk = 1 << 27, (40 if kv else 24)
+        metric = f"256-bucket multisplit, {n} uint64 keys{' + uint64 values' if kv else ''}"
+    elif w == "wide_sort8_kv":
+        n, bpk = 1 << 25, 8 * 40
+        metric = f"key-value radix sort (8-bit digits), {n} uint64 keys + uint64 values"
     else:
         raise SystemExit(f"unknown workload {w!r}")
-    kin = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=dev, generator=g)
+    wide = w.startswith("wide")
+    kdt = torch.int64 if wide else torch.int32
+    kin = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=kdt, device=dev, generator=g)
+    if wide:
+        kin = kin * (1 << 32) + torch.randint(0, 1 << 32, (n,), dtype=kdt, device=dev, generator=g)
     kout = torch.empty_like(kin)
     if w.startswith("sort"):
         vin = torch.arange(n, dtype=torch.int32, device=dev) if kv else None
@@ -282,6 +297,30 @@ def bench_aux(args, torch, gms, W, dev):
                                   None, cnt.data_ptr(), stream)
             if st != 0:
                 raise gms.GmsError(st, "gms_compaction")
+    elif w in ("wide_kv_256", "wide_ko_256"):
+        ctx = gms.multisplit_context(256)
+        gms.multisplit_allocate_key_value(n, ctx)
+        vin = torch.arange(n, dtype=torch.int64, device=dev) if kv else None
+        vout = torch.empty_like(kin) if kv else None
+        desc = gms.range_bucket(256).desc64()
+        vp, vop = (vin.data_ptr(), vout.data_ptr()) if kv else (None, None)
+
+        def step():
+            st = L.gms_multisplit_wide(ctx.handle, kin.data_ptr(), vp, kout.data_ptr(), vop, n, 8,
+                                       8 if kv else 0, desc, None, None, stream)
+            if st != 0:
+                raise gms.GmsError(st, "gms_multisplit_wide")
+    elif w == "wide_sort8_kv":
+        ctx = gms.ms_sort_context()
+        gms.multisplit_sort_8bit_allocate_pairs(n, ctx, key_bytes=8, value_bytes=8)
+        vin = torch.arange(n, dtype=torch.int64, device=dev)
+        vout = torch.empty_like(vin)
+
+        def step():
+            st = L.gms_sort_wide(ctx.handle, kin.data_ptr(), kout.data_ptr(), vin.data_ptr(),
+                                 vout.data_ptr(), n, 8, 8, stream)
+            if st != 0:
+                raise gms.GmsError(st, "gms_sort_wide")
     else:
         def step():
             gms.copy_u32(kin, kout, n)
@@ -295,7 +334,8 @@ def bench_aux(args, torch, gms, W, dev):
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     out = {"metric": metric, "value": round(n / ms / 1e6, 3), "unit": "Gkeys/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "u64" if wide else "u32",
            "data": "synthetic (uniform keys)", "config": {"workload": w, "n": n},
            "call_hbm_fraction": round(n * bpk / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "algorithmic_bytes_per_key": bpk}

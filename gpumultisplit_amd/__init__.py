@@ -52,6 +52,10 @@ class gms_bucket_desc(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("param", ctypes.c_uint32), ("width", ctypes.c_uint32)]
 
 
+class gms_bucket_desc64(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_uint32), ("width", ctypes.c_uint32), ("param", ctypes.c_uint64)]
+
+
 class gms_batch_item(ctypes.Structure):
     _fields_ = [("d_key_in", ctypes.c_void_p), ("d_value_in", ctypes.c_void_p),
                 ("d_key_out", ctypes.c_void_p), ("d_value_out", ctypes.c_void_p),
@@ -95,6 +99,7 @@ _SIGS = [
     ("gms_status_string", ctypes.c_char_p, [_I]),
     ("gms_abi_version", _U32, []),
     ("gms_bucket_ids_host", _I, [_U32, gms_bucket_desc, _P, _P, ctypes.c_size_t]),
+    ("gms_bucket_ids64_host", _I, [_U32, gms_bucket_desc64, _P, _P, ctypes.c_size_t]),
     ("gms_multisplit_create", _I, [_U32, _PP]),
     ("gms_multisplit_allocate", _I, [_P, _U32, _I]),
     ("gms_multisplit_release", _I, [_P]),
@@ -102,6 +107,8 @@ _SIGS = [
     ("gms_multisplit_key_only", _I, [_P, _P, _P, _U32, gms_bucket_desc, _P, _P]),
     ("gms_multisplit_key_value", _I, [_P, _P, _P, _P, _P, _U32, gms_bucket_desc, _P, _P]),
     ("gms_multisplit", _I, [_P, _P, _P, _P, _P, _U32, gms_bucket_desc, _P, _P]),
+    ("gms_multisplit_wide", _I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, gms_bucket_desc64, _P, _P,
+                                 _P]),
     ("gms_subproblem_size", _U32, [_U32, _I]),
     ("gms_tile_keys", _U32, [_U32, _I, _U32]),
     ("gms_multisplit_set_profiling", _I, [_P, _I]),
@@ -115,6 +122,8 @@ _SIGS = [
     ("gms_sort_release", _I, [_P]),
     ("gms_sort_destroy", _I, [_P]),
     ("gms_sort", _I, [_P, _P, _P, _P, _P, _U32, _U32, _P]),
+    ("gms_sort_allocate_wide", _I, [_P, _U32, _U32, _U32, _U32]),
+    ("gms_sort_wide", _I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, _P]),
     ("gms_compaction_create", _I, [_PP]),
     ("gms_compaction_allocate", _I, [_P, _U32, _I]),
     ("gms_compaction_release", _I, [_P]),
@@ -181,8 +190,9 @@ def _check(status: int, what: str) -> None:
         raise GmsError(status, what)
 
 
-def _ptr(x):
-    """Device pointer of a torch tensor (or an int address); None stays None."""
+def _ptr(x, sizes=(4,)):
+    """Device pointer of a torch tensor (or an int address); None stays None.
+    Tensors must have elements of one of `sizes` bytes."""
     if x is None:
         return None
     if isinstance(x, int):
@@ -190,10 +200,23 @@ def _ptr(x):
     if hasattr(x, "data_ptr"):
         if hasattr(x, "is_cuda") and not x.is_cuda:
             raise ValueError("device buffer expected, got a host tensor")
-        if x.element_size() != 4:
-            raise ValueError("32-bit elements expected")
+        if x.element_size() not in sizes:
+            raise ValueError(f"elements of {' or '.join(str(8 * b) for b in sizes)} bits expected")
         return x.data_ptr()
     raise TypeError(f"cannot take a device pointer of {type(x)!r}")
+
+
+def _elem_bytes(*tensors):
+    """Element size shared by the given tensors (4 for raw addresses and None)."""
+    sizes = {t.element_size() for t in tensors if t is not None and hasattr(t, "element_size")}
+    if len(sizes) > 1:
+        raise ValueError("input and output buffers must have the same element size")
+    return sizes.pop() if sizes else 4
+
+
+def _is_wide(*tensors):
+    return any(t is not None and hasattr(t, "element_size") and t.element_size() == 8
+               for t in tensors)
 
 
 def _stream(*tensors):
@@ -219,7 +242,12 @@ class _Bucket:
     width = 0
 
     def desc(self) -> gms_bucket_desc:
+        if self.param > 0xFFFFFFFF:
+            raise ValueError("parameter needs 64 bits: use it with 64-bit keys")
         return gms_bucket_desc(self.kind, self.param, self.width)
+
+    def desc64(self) -> gms_bucket_desc64:
+        return gms_bucket_desc64(self.kind, self.width, self.param)
 
 
 class range_bucket(_Bucket):
@@ -396,12 +424,27 @@ def _offsets_buffer(multisplit_offset, num_buckets):
     raise TypeError("multisplit_offset must be a host numpy array or CPU tensor")
 
 
+def _multisplit_wide(d_key_in, d_value_in, d_key_out, d_value_out, num_elements, context,
+                     bucket_identifier, h_off, d_off, stream, what):
+    kb = _elem_bytes(d_key_in, d_key_out)
+    vb = 0 if d_value_in is None and d_value_out is None else _elem_bytes(d_value_in, d_value_out)
+    _check(lib().gms_multisplit_wide(context.handle, _ptr(d_key_in, (4, 8)), _ptr(d_value_in, (4, 8)),
+                                     _ptr(d_key_out, (4, 8)), _ptr(d_value_out, (4, 8)),
+                                     int(num_elements), kb, vb, bucket_identifier.desc64(), d_off,
+                                     h_off, stream), what)
+
+
 def multisplit_key_only(d_key_in, d_key_out, num_elements: int, context: multisplit_context,
                         bucket_identifier, multisplit_offset=None) -> None:
-    """Stable key-only multisplit (src/api/multisplit.cuh:81-197)."""
+    """Stable key-only multisplit (src/api/multisplit.cuh:81-197); 32- or
+    64-bit keys (the reference's KeyT)."""
     _check_len("d_key_in", d_key_in, num_elements)
     _check_len("d_key_out", d_key_out, num_elements)
     h_off, _ = _offsets_buffer(multisplit_offset, context.num_buckets)
+    if _is_wide(d_key_in, d_key_out):
+        _multisplit_wide(d_key_in, None, d_key_out, None, num_elements, context, bucket_identifier,
+                         h_off, None, _stream(d_key_in, d_key_out), "multisplit_key_only")
+        return
     _check(lib().gms_multisplit_key_only(context.handle, _ptr(d_key_in), _ptr(d_key_out),
                                          int(num_elements), bucket_identifier.desc(), h_off,
                                          _stream(d_key_in, d_key_out)), "multisplit_key_only")
@@ -410,11 +453,17 @@ def multisplit_key_only(d_key_in, d_key_out, num_elements: int, context: multisp
 def multisplit_key_value(d_key_in, d_value_in, d_key_out, d_value_out, num_elements: int,
                          context: multisplit_context, bucket_identifier,
                          multisplit_offset=None) -> None:
-    """Stable key-value multisplit (src/api/multisplit.cuh:199-313)."""
+    """Stable key-value multisplit (src/api/multisplit.cuh:199-313); keys and
+    values of 32 or 64 bits (the reference's KeyT, ValueT)."""
     for name, t in (("d_key_in", d_key_in), ("d_value_in", d_value_in),
                     ("d_key_out", d_key_out), ("d_value_out", d_value_out)):
         _check_len(name, t, num_elements)
     h_off, _ = _offsets_buffer(multisplit_offset, context.num_buckets)
+    if _is_wide(d_key_in, d_value_in, d_key_out, d_value_out):
+        _multisplit_wide(d_key_in, d_value_in, d_key_out, d_value_out, num_elements, context,
+                         bucket_identifier, h_off, None, _stream(d_key_in, d_key_out),
+                         "multisplit_key_value")
+        return
     _check(lib().gms_multisplit_key_value(context.handle, _ptr(d_key_in), _ptr(d_value_in),
                                           _ptr(d_key_out), _ptr(d_value_out), int(num_elements),
                                           bucket_identifier.desc(), h_off,
@@ -425,11 +474,14 @@ def multisplit(d_key_in, d_key_out, num_elements: int, context: multisplit_conte
                bucket_identifier, d_value_in=None, d_value_out=None, d_offsets=None,
                stream=None) -> None:
     """Asynchronous form: optional values, bucket starts into device memory."""
+    st = stream if stream is not None else _stream(d_key_in, d_key_out)
+    if _is_wide(d_key_in, d_value_in, d_key_out, d_value_out):
+        _multisplit_wide(d_key_in, d_value_in, d_key_out, d_value_out, num_elements, context,
+                         bucket_identifier, None, _ptr(d_offsets), st, "gms_multisplit_wide")
+        return
     _check(lib().gms_multisplit(context.handle, _ptr(d_key_in), _ptr(d_value_in), _ptr(d_key_out),
                                 _ptr(d_value_out), int(num_elements), bucket_identifier.desc(),
-                                _ptr(d_offsets),
-                                stream if stream is not None else _stream(d_key_in, d_key_out)),
-           "gms_multisplit")
+                                _ptr(d_offsets), st), "gms_multisplit")
 
 
 def set_profiling(context: multisplit_context, enable: bool) -> None:
@@ -488,19 +540,32 @@ def multisplit_sort_release_memory(context: ms_sort_context) -> None:
 
 
 def _sort_api(bits: int):
-    def allocate(n_elements: int, context: ms_sort_context) -> None:
-        _check(lib().gms_sort_allocate(context.handle, int(n_elements), bits, 0),
-               f"multisplit_sort_{bits}bit_allocate")
+    def allocate(n_elements: int, context: ms_sort_context, key_bytes: int = 4) -> None:
+        if key_bytes == 4:
+            st = lib().gms_sort_allocate(context.handle, int(n_elements), bits, 0)
+        else:
+            st = lib().gms_sort_allocate_wide(context.handle, int(n_elements), bits, key_bytes, 0)
+        _check(st, f"multisplit_sort_{bits}bit_allocate")
         context.digit_bits = bits
 
-    def allocate_pairs(n_elements: int, context: ms_sort_context) -> None:
-        _check(lib().gms_sort_allocate(context.handle, int(n_elements), bits, 1),
-               f"multisplit_sort_{bits}bit_allocate_pairs")
+    def allocate_pairs(n_elements: int, context: ms_sort_context, key_bytes: int = 4,
+                       value_bytes: int = 4) -> None:
+        if key_bytes == 4 and value_bytes == 4:
+            st = lib().gms_sort_allocate(context.handle, int(n_elements), bits, 1)
+        else:
+            st = lib().gms_sort_allocate_wide(context.handle, int(n_elements), bits, key_bytes,
+                                              value_bytes)
+        _check(st, f"multisplit_sort_{bits}bit_allocate_pairs")
         context.digit_bits = bits
 
     def sort(d_key_in, d_key_out, n_elements: int, context: ms_sort_context) -> None:
         _check_len("d_key_in", d_key_in, n_elements)
         _check_len("d_key_out", d_key_out, n_elements)
+        if _is_wide(d_key_in, d_key_out):
+            _check(lib().gms_sort_wide(context.handle, _ptr(d_key_in, (8,)), _ptr(d_key_out, (8,)),
+                                       None, None, int(n_elements), 8, 0,
+                                       _stream(d_key_in, d_key_out)), f"multisplit_sort_{bits}bit")
+            return
         _check(lib().gms_sort(context.handle, _ptr(d_key_in), _ptr(d_key_out), None, None,
                               int(n_elements), bits, _stream(d_key_in, d_key_out)),
                f"multisplit_sort_{bits}bit")
@@ -510,6 +575,13 @@ def _sort_api(bits: int):
         for name, t in (("d_key_in", d_key_in), ("d_key_out", d_key_out),
                         ("d_value_in", d_value_in), ("d_value_out", d_value_out)):
             _check_len(name, t, n_elements)
+        if _is_wide(d_key_in, d_key_out, d_value_in, d_value_out):
+            kb, vb = _elem_bytes(d_key_in, d_key_out), _elem_bytes(d_value_in, d_value_out)
+            _check(lib().gms_sort_wide(context.handle, _ptr(d_key_in, (4, 8)), _ptr(d_key_out, (4, 8)),
+                                       _ptr(d_value_in, (4, 8)), _ptr(d_value_out, (4, 8)),
+                                       int(n_elements), kb, vb, _stream(d_key_in, d_key_out)),
+                   f"multisplit_sort_{bits}bit_pairs")
+            return
         _check(lib().gms_sort(context.handle, _ptr(d_key_in), _ptr(d_key_out), _ptr(d_value_in),
                               _ptr(d_value_out), int(n_elements), bits,
                               _stream(d_key_in, d_key_out)), f"multisplit_sort_{bits}bit_pairs")

@@ -23,6 +23,8 @@ struct gms_sort_ctx {
     uint32_t capacity = 0;
     uint32_t digit_bits = 0;
     int kv = 0;
+    uint32_t key_words = 1;    // widths the temporaries hold (gms_sort_allocate_wide)
+    uint32_t value_words = 1;
     int allocated = 0;
     Scratch s;
     uint32_t* ktmp = nullptr;
@@ -50,7 +52,9 @@ size_t matrix_elems(uint32_t n, uint32_t num_buckets) {
     const int lb = ceil_log2_u32(num_buckets);
     const size_t t_ko = ((size_t)n + tile_keys(lb, false, 0) - 1) / tile_keys(lb, false, 0);
     const size_t t_kv = ((size_t)n + tile_keys(lb, true, 0) - 1) / tile_keys(lb, true, 0);
-    return ((size_t)1 << lb) * std::max(t_ko, t_kv);
+    // the shortest tile of the 64-bit kernels (gms_multisplit_wide, gms_sort_wide)
+    const size_t t_w = ((size_t)n + wide_tile_elems(2, 2) - 1) / wide_tile_elems(2, 2);
+    return ((size_t)1 << lb) * std::max(std::max(t_ko, t_kv), t_w);
 }
 
 gms_status scratch_alloc(Scratch& s, size_t hist_elems) {
@@ -141,6 +145,26 @@ gms_status check_desc(uint32_t num_buckets, gms_bucket_desc d) {
         default: return GMS_ERR_INVALID_ARG;
     }
 }
+
+static gms_status check_desc64(uint32_t B, const gms_bucket_desc64& d, uint32_t key_bits) {
+    switch (d.kind) {
+        case GMS_BUCKET_RANGE:
+        case GMS_BUCKET_IDENTITY: return GMS_SUCCESS;
+        case GMS_BUCKET_DELTA:
+            return (d.param >= 1 && (key_bits == 64 || d.param <= 0xFFFFFFFFull)) ? GMS_SUCCESS
+                                                                                  : GMS_ERR_INVALID_ARG;
+        case GMS_BUCKET_DIGIT:
+            if (d.param >= key_bits || d.width > 20) return GMS_ERR_INVALID_ARG;
+            if (d.width == 0) return is_pow2(B) ? GMS_SUCCESS : GMS_ERR_INVALID_ARG;
+            return (1u << d.width) <= B ? GMS_SUCCESS : GMS_ERR_INVALID_ARG;
+        case GMS_BUCKET_LOWBITS:
+            return (d.param < key_bits && d.param < 32 && (1u << d.param) <= B) ? GMS_SUCCESS
+                                                                                : GMS_ERR_INVALID_ARG;
+        default: return GMS_ERR_INVALID_ARG;
+    }
+}
+
+static bool aligned8(const void* p) { return (((uintptr_t)p) & 7u) == 0; }
 
 static uint32_t digit_mask(uint32_t B, gms_bucket_desc d) {
     return d.width ? (1u << d.width) - 1u : B - 1u;
@@ -287,6 +311,17 @@ gms_status gms_bucket_ids_host(uint32_t num_buckets, gms_bucket_desc desc, const
     return GMS_SUCCESS;
 }
 
+gms_status gms_bucket_ids64_host(uint32_t num_buckets, gms_bucket_desc64 desc,
+                                 const uint64_t* keys, uint32_t* ids, size_t n) {
+    if (num_buckets < 1 || num_buckets > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    gms_status st = check_desc64(num_buckets, desc, 64);
+    if (st != GMS_SUCCESS) return st;
+    if (n && (!keys || !ids)) return GMS_ERR_INVALID_ARG;
+    const AnyF64 f = AnyF64::make(desc.kind, desc.param, desc.width, num_buckets);
+    for (size_t i = 0; i < n; ++i) ids[i] = num_buckets == 1 ? 0u : f(keys[i]);
+    return GMS_SUCCESS;
+}
+
 uint32_t gms_subproblem_size(uint32_t num_buckets, int key_value) {
     if (num_buckets < 1 || num_buckets > 256) return 0;
     return tile_keys(ceil_log2_u32(num_buckets), key_value != 0, 0);
@@ -409,6 +444,63 @@ gms_status gms_multisplit_key_value(gms_multisplit_ctx* ctx, const uint32_t* kin
     return multisplit_host_offsets(ctx, kin, vin, kout, vout, n, d, h_offsets, stream);
 }
 
+
+
+gms_status gms_multisplit_wide(gms_multisplit_ctx* ctx, const void* kin, const void* vin,
+                               void* kout, void* vout, uint32_t n, uint32_t key_bytes,
+                               uint32_t value_bytes, gms_bucket_desc64 d, uint32_t* d_offsets,
+                               uint32_t* h_offsets, void* stream) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    if ((key_bytes != 4 && key_bytes != 8) || (value_bytes != 0 && value_bytes != 4 && value_bytes != 8))
+        return GMS_ERR_INVALID_ARG;
+    if (value_bytes && n && (!vin || !vout)) return GMS_ERR_INVALID_ARG;
+    if (!value_bytes && (vin || vout)) return GMS_ERR_INVALID_ARG;
+    const uint32_t kw = key_bytes / 4, vw = value_bytes / 4;
+    hipStream_t s = (hipStream_t)stream;
+    gms_status st;
+    if (kw == 1 && vw <= 1) {  // the 32-bit path, whose identifiers take 32-bit parameters
+        st = check_desc64(ctx->num_buckets, d, 32);
+        if (st != GMS_SUCCESS) return st;
+        const gms_bucket_desc d32{d.kind, (uint32_t)d.param, d.width};
+        const uint32_t* k = static_cast<const uint32_t*>(kin);
+        const uint32_t* v = static_cast<const uint32_t*>(vin);
+        uint32_t* ko = static_cast<uint32_t*>(kout);
+        uint32_t* vo = static_cast<uint32_t*>(vout);
+        if (h_offsets) {
+            st = multisplit_host_offsets(ctx, k, v, ko, vo, n, d32, h_offsets, stream);
+            if (st != GMS_SUCCESS || !d_offsets) return st;
+            return hipMemcpyAsync(d_offsets, h_offsets, (size_t)ctx->num_buckets * 4,
+                                  hipMemcpyHostToDevice, s) == hipSuccess ? GMS_SUCCESS : GMS_ERR_HIP;
+        }
+        return gms_multisplit(ctx, k, v, ko, vo, n, d32, d_offsets, stream);
+    }
+    if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
+    if (n > ctx->capacity) return GMS_ERR_CAPACITY;
+    if (ctx->num_buckets > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    st = check_desc64(ctx->num_buckets, d, 32 * kw);
+    if (st != GMS_SUCCESS) return st;
+    if (n) {
+        if (!kin || !kout) return GMS_ERR_INVALID_ARG;
+        if ((kw == 2 && (!aligned8(kin) || !aligned8(kout))) ||
+            (vw == 2 && (!aligned8(vin) || !aligned8(vout))))
+            return GMS_ERR_INVALID_ARG;
+        const size_t kb = (size_t)n * key_bytes, vb = (size_t)n * value_bytes;
+        if (ranges_overlap(kin, kb, kout, kb)) return GMS_ERR_ALIAS;
+        if (vw && (ranges_overlap(vin, vb, vout, vb) || ranges_overlap(kin, kb, vout, vb) ||
+                   ranges_overlap(vin, vb, kout, kb) || ranges_overlap(kout, kb, vout, vb)))
+            return GMS_ERR_ALIAS;
+    }
+    uint32_t* offs = d_offsets ? d_offsets : (h_offsets ? ctx->s.offs : nullptr);
+    st = run_wide_multisplit(ctx->s, static_cast<const uint32_t*>(kin),
+                             static_cast<const uint32_t*>(vin), static_cast<uint32_t*>(kout),
+                             static_cast<uint32_t*>(vout), n, kw, vw, ctx->num_buckets, d, offs, s);
+    if (st != GMS_SUCCESS || !h_offsets) return st;
+    if (hipMemcpyAsync(h_offsets, offs, (size_t)ctx->num_buckets * sizeof(uint32_t),
+                       hipMemcpyDeviceToHost, s) != hipSuccess)
+        return GMS_ERR_HIP;
+    return hip_status(hipStreamSynchronize(s));
+}
+
 gms_status gms_multisplit_set_profiling(gms_multisplit_ctx* ctx, int enable) {
     if (!ctx) return GMS_ERR_INVALID_ARG;
     ctx->prof.on = enable != 0;
@@ -504,6 +596,42 @@ gms_status gms_sort_allocate(gms_sort_ctx* ctx, uint32_t n_capacity, uint32_t di
     ctx->capacity = n_capacity;
     ctx->digit_bits = digit_bits;
     ctx->kv = key_value != 0;
+    ctx->key_words = 1;
+    ctx->value_words = key_value ? 1u : 0u;
+    ctx->allocated = 1;
+    return GMS_SUCCESS;
+}
+
+gms_status gms_sort_allocate_wide(gms_sort_ctx* ctx, uint32_t n_capacity, uint32_t digit_bits,
+                                  uint32_t key_bytes, uint32_t value_bytes) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    if ((key_bytes != 4 && key_bytes != 8) || (value_bytes != 0 && value_bytes != 4 && value_bytes != 8))
+        return GMS_ERR_INVALID_ARG;
+    const uint32_t kw = key_bytes / 4, vw = value_bytes / 4;
+    SortPass p[16];
+    const int np = wide_sort_passes(kw, digit_bits, p);
+    if (np == 0) return GMS_ERR_INVALID_ARG;
+    gms_sort_release(ctx);
+    size_t me = 0;
+    for (int i = 0; i < np; ++i) me = std::max(me, matrix_elems(n_capacity, 1u << p[i].bits));
+    gms_status st = scratch_alloc(ctx->s, me);
+    if (st != GMS_SUCCESS) return st;
+    const size_t words = std::max<size_t>((size_t)n_capacity, 1);
+    if (hipMalloc((void**)&ctx->ktmp, words * kw * sizeof(uint32_t)) != hipSuccess) {
+        ctx->ktmp = nullptr;
+        gms_sort_release(ctx);
+        return GMS_ERR_ALLOC;
+    }
+    if (vw && hipMalloc((void**)&ctx->vtmp, words * vw * sizeof(uint32_t)) != hipSuccess) {
+        ctx->vtmp = nullptr;
+        gms_sort_release(ctx);
+        return GMS_ERR_ALLOC;
+    }
+    ctx->capacity = n_capacity;
+    ctx->digit_bits = digit_bits;
+    ctx->kv = vw != 0;
+    ctx->key_words = kw;
+    ctx->value_words = vw;
     ctx->allocated = 1;
     return GMS_SUCCESS;
 }
@@ -550,6 +678,38 @@ gms_status gms_sort(gms_sort_ctx* ctx, const uint32_t* kin, uint32_t* kout, cons
         sv = dv;
     }
     return GMS_SUCCESS;
+}
+
+gms_status gms_sort_wide(gms_sort_ctx* ctx, const void* kin, void* kout, const void* vin,
+                         void* vout, uint32_t n, uint32_t key_bytes, uint32_t value_bytes,
+                         void* stream) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
+    if ((key_bytes != 4 && key_bytes != 8) || (value_bytes != 0 && value_bytes != 4 && value_bytes != 8))
+        return GMS_ERR_INVALID_ARG;
+    const uint32_t kw = key_bytes / 4, vw = value_bytes / 4;
+    if (kw == 1 && vw <= 1)
+        return gms_sort(ctx, static_cast<const uint32_t*>(kin), static_cast<uint32_t*>(kout),
+                        static_cast<const uint32_t*>(vin), static_cast<uint32_t*>(vout), n,
+                        ctx->digit_bits, stream);
+    if (n > ctx->capacity) return GMS_ERR_CAPACITY;
+    if (kw > ctx->key_words || vw > ctx->value_words) return GMS_ERR_NOT_ALLOCATED;
+    if (value_bytes && n && (!vin || !vout)) return GMS_ERR_INVALID_ARG;
+    if (!value_bytes && (vin || vout)) return GMS_ERR_INVALID_ARG;
+    if (n == 0) return GMS_SUCCESS;
+    if (!kin || !kout) return GMS_ERR_INVALID_ARG;
+    if ((kw == 2 && (!aligned8(kin) || !aligned8(kout))) ||
+        (vw == 2 && (!aligned8(vin) || !aligned8(vout))))
+        return GMS_ERR_INVALID_ARG;
+    const size_t kb = (size_t)n * key_bytes, vb = (size_t)n * value_bytes;
+    if (ranges_overlap(kin, kb, kout, kb) ||
+        (vw && (ranges_overlap(vin, vb, vout, vb) || ranges_overlap(kin, kb, vout, vb) ||
+                ranges_overlap(vin, vb, kout, kb) || ranges_overlap(kout, kb, vout, vb))))
+        return GMS_ERR_ALIAS;
+    return run_wide_sort(ctx->s, ctx->ktmp, ctx->vtmp, static_cast<const uint32_t*>(kin),
+                         static_cast<const uint32_t*>(vin), static_cast<uint32_t*>(kout),
+                         static_cast<uint32_t*>(vout), n, kw, vw, ctx->digit_bits,
+                         (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------- compaction

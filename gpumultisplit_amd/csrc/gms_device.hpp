@@ -124,6 +124,52 @@ struct AnyF {
     }
 };
 
+// The identifiers for 64-bit keys, with the same branch-free form as AnyF:
+//   t = mulhi64(k, m);  q = (t + (((k - t) >> sh1) & em)) >> sh2;
+//   id = min(q & mask, bmax)
+// range:    m = B, em = 0                        -> q = floor(k*B / 2^64)
+// delta:    64-bit Granlund-Montgomery constants (the 64-bit case of DeltaF's
+//           construction, same paper): l = ceil(log2 d),
+//           m = floor(2^64 (2^l - d) / d) + 1, sh1 = min(l, 1), sh2 = max(l-1, 0)
+//                                                -> q = floor(k / d)
+// digit:    m = 0, sh2 = shift (< 64), mask = 2^width - 1 (or B - 1)
+// identity: m = 0                                -> q = k
+// low bits: m = 0, mask = 2^param - 1
+struct AnyF64 {
+    uint64_t m, em, mask;
+    uint32_t sh1, sh2, bmax;
+    GMS_HD uint32_t operator()(uint64_t k) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        const uint64_t t = __umul64hi(k, m);
+#else
+        const uint64_t t = (uint64_t)(((unsigned __int128)k * m) >> 64);
+#endif
+        const uint64_t q = ((t + (((k - t) >> sh1) & em)) >> sh2) & mask;
+        return q < bmax ? (uint32_t)q : bmax;
+    }
+    // kind numbering of gms_bucket_kind; arguments are assumed validated
+    // (delta >= 1, shifts < 64).
+    static AnyF64 make(uint32_t kind, uint64_t param, uint32_t width, uint32_t num_buckets) {
+        const uint32_t bmax = num_buckets - 1u;
+        switch (kind) {
+            case 0: return AnyF64{num_buckets, 0ull, ~0ull, 0u, 0u, bmax};
+            case 1: {
+                uint32_t l = 0;
+                while (l < 64 && ((unsigned __int128)1 << l) < (unsigned __int128)param) ++l;
+                const unsigned __int128 two_l = (unsigned __int128)1 << l;
+                const uint64_t m = (uint64_t)((((two_l - param) << 64) / param) + 1);
+                return AnyF64{m, ~0ull, ~0ull, l < 1 ? l : 1u, l > 0 ? l - 1u : 0u, bmax};
+            }
+            case 2: {
+                const uint64_t mask = width ? (1ull << width) - 1ull : (uint64_t)bmax;
+                return AnyF64{0ull, ~0ull, mask, 0u, (uint32_t)param, bmax};
+            }
+            case 4: return AnyF64{0ull, ~0ull, (1ull << param) - 1ull, 0u, 0u, bmax};
+            default: return AnyF64{0ull, ~0ull, ~0ull, 0u, 0u, bmax};
+        }
+    }
+};
+
 // One digit of another identifier's bucket id: (base(k) >> shift) & mask,
 // with base's clamp applied first.  The LSD passes of a multisplit into more
 // than 256 buckets rank by these digits of the (up to 20-bit) bucket id.

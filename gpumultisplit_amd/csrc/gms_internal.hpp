@@ -128,6 +128,21 @@ struct SortPass {
 };
 int sort_schedule(uint32_t digit_bits, SortPass* p);
 
+// Elements with 64-bit keys and/or values (gms_wide.hip): key_words 1 or 2,
+// value_words 0, 1 or 2, not both below 2; up to 256 buckets.
+uint32_t wide_tile_elems(uint32_t key_words, uint32_t value_words);
+gms_status run_wide_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin,
+                               uint32_t* kout, uint32_t* vout, uint32_t n, uint32_t key_words,
+                               uint32_t value_words, uint32_t num_buckets,
+                               const gms_bucket_desc64& d, uint32_t* d_offsets,
+                               hipStream_t stream);
+// Passes of the radix sort of key_words-word keys (p holds up to 16).
+int wide_sort_passes(uint32_t key_words, uint32_t digit_bits, SortPass* p);
+gms_status run_wide_sort(Scratch& s, uint32_t* ktmp, uint32_t* vtmp, const uint32_t* kin,
+                         const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint32_t n,
+                         uint32_t key_words, uint32_t value_words, uint32_t digit_bits,
+                         hipStream_t stream);
+
 gms_status hip_status(hipError_t e);
 bool ranges_overlap(const void* a, size_t an, const void* b, size_t bn);
 

@@ -60,6 +60,21 @@ typedef struct gms_bucket_desc {
     uint32_t width;   /* digit width in bits (GMS_BUCKET_DIGIT only) */
 } gms_bucket_desc;
 
+/* Identifier of a 64-bit key (or of a 32-bit key whose values are 64-bit):
+ * the same kinds, with a 64-bit delta / shift / flag.  On 64-bit keys
+ *   RANGE    floor(key * B / 2^64)
+ *   DELTA    key / param
+ *   DIGIT    (key >> param) & (2^width - 1), or & (B - 1) when width is 0
+ *   IDENTITY key
+ *   LOWBITS  key & (2^param - 1)
+ * each clamped into [0, B), as the 32-bit forms (the reference's functors
+ * written for KeyT = uint64_t, src/main/main_multisplit.cu:34-47). */
+typedef struct gms_bucket_desc64 {
+    uint32_t kind;    /* gms_bucket_kind */
+    uint32_t width;   /* digit width in bits (GMS_BUCKET_DIGIT only) */
+    uint64_t param;   /* delta, shift or flag */
+} gms_bucket_desc64;
+
 /* Short description of a status code (static storage). */
 const char *gms_status_string(gms_status s);
 
@@ -70,6 +85,9 @@ uint32_t gms_abi_version(void);
  * (used to check the magic-number division of GMS_BUCKET_DELTA on the CPU). */
 gms_status gms_bucket_ids_host(uint32_t num_buckets, gms_bucket_desc desc,
                                const uint32_t *keys, uint32_t *ids, size_t n);
+/* The same for 64-bit keys and gms_bucket_desc64 (up to 256 buckets). */
+gms_status gms_bucket_ids64_host(uint32_t num_buckets, gms_bucket_desc64 desc,
+                                 const uint64_t *keys, uint32_t *ids, size_t n);
 
 /* ------------------------------------------------------------------------ */
 /* Multisplit                                                                */
@@ -122,6 +140,21 @@ gms_status gms_multisplit(gms_multisplit_ctx *ctx, const uint32_t *d_key_in,
                           const uint32_t *d_value_in, uint32_t *d_key_out,
                           uint32_t *d_value_out, uint32_t n, gms_bucket_desc desc,
                           uint32_t *d_offsets, void *stream);
+
+/* Multisplit of 32- or 64-bit keys carrying 32- or 64-bit values: the
+ * reference's multisplit_key_only<KeyT> / multisplit_key_value<KeyT, ValueT>
+ * (src/api/multisplit.cuh:81-197, :199-313) for KeyT, ValueT of 4 or 8 bytes.
+ * key_bytes 4 or 8; value_bytes 0 (key-only), 4 or 8.  Arrays of 8-byte
+ * elements must be 8-byte aligned.  With 4-byte keys and values (or none) this
+ * is gms_multisplit / gms_multisplit_key_value; any 8-byte operand takes the
+ * 64-bit kernels, which support up to 256 buckets.  Bucket starts go to
+ * d_offsets (device, asynchronous) and/or h_offsets (host; the call then
+ * blocks), num_buckets entries each.  The context's allocation serves both. */
+gms_status gms_multisplit_wide(gms_multisplit_ctx *ctx, const void *d_key_in,
+                               const void *d_value_in, void *d_key_out, void *d_value_out,
+                               uint32_t n, uint32_t key_bytes, uint32_t value_bytes,
+                               gms_bucket_desc64 desc, uint32_t *d_offsets, uint32_t *h_offsets,
+                               void *stream);
 
 /* subproblem_size_bms_key_only / _key_value (src/api/bms_api.h:23-89): keys
  * per tile for a bucket count (the tile of inputs below 2^26 keys, also the
@@ -183,6 +216,20 @@ gms_status gms_sort_destroy(gms_sort_ctx *ctx);
 gms_status gms_sort(gms_sort_ctx *ctx, const uint32_t *d_key_in, uint32_t *d_key_out,
                     const uint32_t *d_value_in, uint32_t *d_value_out, uint32_t n,
                     uint32_t digit_bits, void *stream);
+
+/* Radix sort of 64-bit keys and/or 64-bit values (the reference's sort
+ * templates for KeyT = uint64_t / ValueT of 8 bytes, multisplit_sort.cuh:
+ * 366-527, multisplit_sort_pairs.cuh:67-228).  gms_sort_allocate_wide sizes
+ * the context for key_bytes (4 or 8) and value_bytes (0, 4 or 8) with at least
+ * one of 8 bytes; gms_sort_wide then sorts with those widths or narrower.
+ * 64-bit keys take ceil(64 / digit_bits) passes of digit_bits (4..8) bits from
+ * bit 0 up, 32-bit keys the reference's 32-bit schedule.  Stable; d_key_in is
+ * not written; 8-byte arrays must be 8-byte aligned. */
+gms_status gms_sort_allocate_wide(gms_sort_ctx *ctx, uint32_t n_capacity, uint32_t digit_bits,
+                                  uint32_t key_bytes, uint32_t value_bytes);
+gms_status gms_sort_wide(gms_sort_ctx *ctx, const void *d_key_in, void *d_key_out,
+                         const void *d_value_in, void *d_value_out, uint32_t n,
+                         uint32_t key_bytes, uint32_t value_bytes, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Compaction (stable 2-bucket multisplit returning the bucket-0 count)      */

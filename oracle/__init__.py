@@ -62,6 +62,15 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = [u32p, ctypes.c_uint64, u32p, u32p, ctypes.c_uint32]
             fn.restype = None
+        u64 = ctypes.c_uint64
+        L.ms_oracle_bucket_ids64.argtypes = [ctypes.c_uint32, ctypes.c_uint32, u64, ctypes.c_uint32,
+                                             u32p, u64, u32p]
+        L.ms_oracle_bucket_ids64.restype = None
+        L.ms_oracle_multisplit_wide.argtypes = ([ctypes.c_uint32, ctypes.c_uint32, u64] +
+                                                [ctypes.c_uint32] * 3 + [u32p] * 4 + [u64, u32p])
+        L.ms_oracle_multisplit_wide.restype = ctypes.c_int
+        L.ms_oracle_sort_wide.argtypes = [u32p] * 4 + [u64, ctypes.c_uint32, ctypes.c_uint32]
+        L.ms_oracle_sort_wide.restype = ctypes.c_int
         L.ms_oracle_compaction.argtypes = [ctypes.c_uint32] * 2 + [u32p] * 4 + [ctypes.c_uint64]
         L.ms_oracle_compaction.restype = ctypes.c_uint64
         L.ms_oracle_tile_histograms.argtypes = [ctypes.c_uint32] * 3 + [u32p, ctypes.c_uint64,
@@ -163,3 +172,52 @@ def tile_histograms(keys, num_buckets: int, tile: int, kind: int = RANGE, param:
     H = np.zeros(num_buckets * max(nt, 1), dtype=np.uint32)
     lib().ms_oracle_tile_histograms(kind, num_buckets, param, _ptr(keys), keys.size, tile, _ptr(H))
     return H[: num_buckets * nt]
+
+
+def _words(a):
+    """(uint32 word view, words per element) of a uint32/uint64 array."""
+    a = np.ascontiguousarray(a)
+    if a.dtype.itemsize == 8:
+        return np.ascontiguousarray(a.astype(np.uint64)).view(np.uint32), 2
+    return _u32(a), 1
+
+
+def bucket_ids64(keys, num_buckets: int, kind: int = RANGE, param: int = 0, width: int = 0) -> np.ndarray:
+    """Bucket ids of 64-bit keys (the 64-bit forms of the identifiers)."""
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    ids = np.zeros(max(k.size, 1), np.uint32)
+    lib().ms_oracle_bucket_ids64(kind, num_buckets, int(param), width, _ptr(k.view(np.uint32)),
+                                 k.size, _ptr(ids))
+    return ids[:k.size]
+
+
+def multisplit_wide(keys, num_buckets: int, kind: int = RANGE, param: int = 0, values=None,
+                    width: int = 0):
+    """Stable multisplit of uint64 (or uint32) keys carrying uint32/uint64 values.
+    Returns (keys_out, values_out or None, bucket starts)."""
+    kw, nkw = _words(keys)
+    n = kw.size // nkw
+    vw, nvw = (None, 0) if values is None else _words(values)
+    ko = np.zeros(max(kw.size, 1), np.uint32)
+    vo = None if vw is None else np.zeros(max(vw.size, 1), np.uint32)
+    offs = np.zeros(num_buckets, np.uint32)
+    rc = lib().ms_oracle_multisplit_wide(kind, num_buckets, int(param), width, nkw, nvw, _ptr(kw),
+                                         _ptr(vw), _ptr(ko), _ptr(vo), n, _ptr(offs))
+    assert rc == 0
+    kt = np.uint64 if nkw == 2 else np.uint32
+    kout = ko[:kw.size].view(kt)
+    vout = None if vo is None else vo[:vw.size].view(np.uint64 if nvw == 2 else np.uint32)
+    return kout, vout, offs
+
+
+def sort_wide(keys, digit_bits: int = 8, values=None):
+    """Stable LSD sort of uint64 keys (values of 32 or 64 bits follow)."""
+    kw, _ = _words(np.asarray(keys, dtype=np.uint64))
+    n = kw.size // 2
+    vw, nvw = (None, 0) if values is None else _words(values)
+    ko = np.zeros(max(kw.size, 2), np.uint32)
+    vo = None if vw is None else np.zeros(max(vw.size, 1), np.uint32)
+    rc = lib().ms_oracle_sort_wide(_ptr(kw), _ptr(vw), _ptr(ko), _ptr(vo), n, nvw, digit_bits)
+    assert rc == 0
+    vout = None if vo is None else vo[:vw.size].view(np.uint64 if nvw == 2 else np.uint32)
+    return ko[:kw.size].view(np.uint64), vout

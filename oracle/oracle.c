@@ -25,6 +25,10 @@
  *   ms_oracle_histogram_range numpy.searchsorted(side="right") - 1 + bincount,
  *                            incl. samples on the edges, outside, and NaN.
  *   ms_oracle_compaction     Python list partition; count = partition point.
+ *   ms_oracle_bucket_ids64   Python integers (k*B >> 64, k // d, ...) for every
+ *                            kind, incl. keys 0, 2^64-1 and range boundaries.
+ *   ms_oracle_multisplit_wide numpy stable argsort of the 64-bit bucket ids.
+ *   ms_oracle_sort_wide      numpy stable argsort of the 64-bit keys.
  * The reference itself ships no golden vectors or known-answer tests; the
  * fixtures under tests/golden/ are produced by this oracle and cross-checked by
  * the independent checks above when they are generated and again in tests.
@@ -285,4 +289,119 @@ void ms_oracle_tile_histograms(uint32_t kind, uint32_t B, uint32_t param,
     memset(H, 0, (size_t)(B * nt) * sizeof(uint32_t));
     for (uint64_t i = 0; i < n; ++i)
         H[(uint64_t)bucket_of(kind, B, param, keys[i]) * nt + i / tile]++;
+}
+
+/*
+ * 64-bit keys and 64-bit values.  The reference's API is templated on KeyT and
+ * ValueT (src/api/multisplit.cuh:81-82, :199-200; multisplit_sort.cuh:366-527;
+ * multisplit_sort_pairs.cuh:67-228) and its CPU multisplit is generic over the
+ * key type (src/cpu_functions.h:155-288); its bucket identifiers extend to
+ * 64-bit keys as written there: key / delta, (key >> shift) & mask, key.  The
+ * range identifier is floor(key * B / 2^64), the 64-bit form of the 32-bit one.
+ * Elements are key_words (1 or 2) and value_words (0, 1 or 2) little-endian
+ * 32-bit words: a 64-bit key is lo | hi << 32.
+ */
+static inline uint32_t bucket_of64(uint32_t kind, uint32_t B, uint64_t param, uint32_t width,
+                                   uint64_t key)
+{
+    uint64_t b;
+    switch (kind) {
+    case MS_OR_RANGE:    b = (uint64_t)(((unsigned __int128)key * B) >> 64); break;
+    case MS_OR_DELTA:    b = key / param; break;
+    case MS_OR_DIGIT:    b = (key >> param) & (width ? ((1ull << width) - 1ull) : (uint64_t)(B - 1u)); break;
+    case MS_OR_IDENTITY: b = key; break;
+    case MS_OR_LOWBITS:  b = key & (param >= 64u ? ~0ull : ((1ull << param) - 1ull)); break;
+    default:             b = 0; break;
+    }
+    return b < B ? (uint32_t)b : B - 1u;
+}
+
+static inline uint64_t key_at(const uint32_t *kin, uint32_t kw, uint64_t i)
+{
+    return kw == 2 ? ((uint64_t)kin[2 * i] | ((uint64_t)kin[2 * i + 1] << 32)) : kin[i];
+}
+
+/* Bucket of element i: the 64-bit identifiers for 64-bit keys, the 32-bit
+ * ones (bucket_of, with a digit width) for 32-bit keys. */
+static inline uint32_t bucket_at(uint32_t kind, uint32_t B, uint64_t param, uint32_t width,
+                                 const uint32_t *kin, uint32_t kw, uint64_t i)
+{
+    if (kw == 2) return bucket_of64(kind, B, param, width, key_at(kin, 2, i));
+    if (kind == MS_OR_DIGIT && width) {
+        const uint32_t b = (kin[i] >> param) & (width >= 32 ? ~0u : ((1u << width) - 1u));
+        return b < B ? b : B - 1u;
+    }
+    return bucket_of(kind, B, (uint32_t)param, kin[i]);
+}
+
+void ms_oracle_bucket_ids64(uint32_t kind, uint32_t B, uint64_t param, uint32_t width,
+                            const uint64_t *keys, uint64_t n, uint32_t *ids)
+{
+    for (uint64_t i = 0; i < n; ++i)
+        ids[i] = bucket_of64(kind, B, param, width, keys[i]);
+}
+
+/* Stable multisplit of elements of key_words + value_words words; the same
+ * count -> exclusive scan -> stable scatter as ms_oracle_multisplit
+ * (cpu_functions.h:155-288).  vin/vout NULL (or value_words 0) = key-only. */
+int ms_oracle_multisplit_wide(uint32_t kind, uint32_t B, uint64_t param, uint32_t width,
+                              uint32_t key_words, uint32_t value_words,
+                              const uint32_t *kin, const uint32_t *vin,
+                              uint32_t *kout, uint32_t *vout, uint64_t n, uint32_t *offsets)
+{
+    if (key_words < 1 || key_words > 2 || value_words > 2) return -2;
+    uint64_t *count = (uint64_t *)calloc(B, sizeof(uint64_t));
+    uint64_t *cursor = (uint64_t *)calloc(B, sizeof(uint64_t));
+    if (!count || !cursor) { free(count); free(cursor); return -1; }
+    for (uint64_t i = 0; i < n; ++i)
+        count[bucket_at(kind, B, param, width, kin, key_words, i)]++;
+    uint64_t run = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+        cursor[b] = run;
+        run += count[b];
+    }
+    if (offsets)
+        for (uint32_t b = 0; b < B; ++b) offsets[b] = (uint32_t)cursor[b];
+    const int kv = vin && vout && value_words;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t dst = cursor[bucket_at(kind, B, param, width, kin, key_words, i)]++;
+        for (uint32_t w = 0; w < key_words; ++w) kout[dst * key_words + w] = kin[i * key_words + w];
+        if (kv)
+            for (uint32_t w = 0; w < value_words; ++w)
+                vout[dst * value_words + w] = vin[i * value_words + w];
+    }
+    free(count);
+    free(cursor);
+    return 0;
+}
+
+/* Stable LSD radix sort of 64-bit keys (optionally carrying values of
+ * value_words words): passes of digit_bits bits from bit 0 up, the last one
+ * narrower, each a stable multisplit by that digit (the schedule of
+ * multisplit_sort.cuh:320-559 continued over 64 bits). */
+int ms_oracle_sort_wide(const uint32_t *kin, const uint32_t *vin, uint32_t *kout, uint32_t *vout,
+                        uint64_t n, uint32_t value_words, uint32_t digit_bits)
+{
+    if (digit_bits < 1 || digit_bits > 8 || value_words > 2) return -2;
+    const int kv = vin != NULL && vout != NULL && value_words;
+    const size_t kb = (n ? n : 1) * 2 * sizeof(uint32_t), vb = (n ? n : 1) * (value_words ? value_words : 1) * sizeof(uint32_t);
+    uint32_t *ka = (uint32_t *)malloc(kb), *kt = (uint32_t *)malloc(kb);
+    uint32_t *va = kv ? (uint32_t *)malloc(vb) : NULL, *vt = kv ? (uint32_t *)malloc(vb) : NULL;
+    if (!ka || !kt || (kv && (!va || !vt))) {
+        free(ka); free(kt); free(va); free(vt);
+        return -1;
+    }
+    memcpy(ka, kin, n * 2 * sizeof(uint32_t));
+    if (kv) memcpy(va, vin, n * value_words * sizeof(uint32_t));
+    for (uint32_t shift = 0; shift < 64; shift += digit_bits) {
+        const uint32_t w = 64 - shift < digit_bits ? 64 - shift : digit_bits;
+        ms_oracle_multisplit_wide(MS_OR_DIGIT, 1u << w, shift, w, 2, kv ? value_words : 0, ka,
+                                  kv ? va : NULL, kt, kv ? vt : NULL, n, NULL);
+        uint32_t *t = ka; ka = kt; kt = t;
+        if (kv) { t = va; va = vt; vt = t; }
+    }
+    memcpy(kout, ka, n * 2 * sizeof(uint32_t));
+    if (kv) memcpy(vout, va, n * value_words * sizeof(uint32_t));
+    free(ka); free(kt); free(va); free(vt);
+    return 0;
 }

@@ -25,6 +25,12 @@ void ms_oracle_histogram_range_f32(const float* x, uint64_t n, int32_t* hist, co
                                    uint32_t nl);
 void ms_oracle_histogram_range_u32(const uint32_t* x, uint64_t n, int32_t* hist,
                                    const uint32_t* levels, uint32_t nl);
+int ms_oracle_multisplit_wide(uint32_t kind, uint32_t B, uint64_t param, uint32_t width,
+                              uint32_t key_words, uint32_t value_words, const uint32_t* kin,
+                              const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint64_t n,
+                              uint32_t* offsets);
+int ms_oracle_sort_wide(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
+                        uint64_t n, uint32_t value_words, uint32_t digit_bits);
 void ms_oracle_tile_histograms(uint32_t kind, uint32_t B, uint32_t param, const uint32_t* keys,
                                uint64_t n, uint32_t tile, uint32_t* H);
 }
@@ -280,6 +286,48 @@ int main(int argc, char** argv) {
         }
     }
     printf("batch done\n");
+
+    // 64-bit keys / values: exact-size buffers (ASan sees any overrun)
+    for (uint32_t kw : {1u, 2u}) {
+        for (uint32_t vw : {0u, 1u, 2u}) {
+            if (kw == 1 && vw < 2) continue;
+            for (uint32_t n : {0u, 1u, 3072u * 2 + 5, 4096u + 1}) {
+                const uint32_t B = 256;
+                Buf kin(n * kw), vin(n * vw), kout(n * kw), vout(n * vw);
+                for (uint32_t i = 0; i < n * kw; ++i) kin.p[i] = rnd();
+                for (uint32_t i = 0; i < n * vw; ++i) vin.p[i] = i;
+                gms_multisplit_ctx* c;
+                gms_multisplit_create(B, &c);
+                gms_multisplit_allocate(c, n, 1);
+                std::vector<uint32_t> offs(B), wk(n * kw + 1), wv(n * vw + 1), wo(B);
+                EXPECT(gms_multisplit_wide(c, kin.p, vw ? vin.p : nullptr, kout.p,
+                                           vw ? vout.p : nullptr, n, 4 * kw, 4 * vw,
+                                           gms_bucket_desc64{GMS_BUCKET_RANGE, 0, 0}, nullptr,
+                                           offs.data(), nullptr) == GMS_SUCCESS,
+                       "wide status kw=%u vw=%u n=%u", kw, vw, n);
+                ms_oracle_multisplit_wide(0, B, 0, 0, kw, vw, kin.p, vin.p, wk.data(), wv.data(), n,
+                                          wo.data());
+                EXPECT(memcmp(kout.p, wk.data(), (size_t)n * kw * 4) == 0 &&
+                           memcmp(vout.p, wv.data(), (size_t)n * vw * 4) == 0 && offs == wo,
+                       "wide multisplit kw=%u vw=%u n=%u", kw, vw, n);
+                gms_multisplit_destroy(c);
+                if (kw == 2) {
+                    gms_sort_ctx* sc;
+                    gms_sort_create(&sc);
+                    EXPECT(gms_sort_allocate_wide(sc, n, 8, 8, 4 * vw) == GMS_SUCCESS, "sort alloc");
+                    EXPECT(gms_sort_wide(sc, kin.p, kout.p, vw ? vin.p : nullptr,
+                                         vw ? vout.p : nullptr, n, 8, 4 * vw, nullptr) == GMS_SUCCESS,
+                           "sort wide status");
+                    ms_oracle_sort_wide(kin.p, vin.p, wk.data(), wv.data(), n, vw, 8);
+                    EXPECT(memcmp(kout.p, wk.data(), (size_t)n * 8) == 0 &&
+                               memcmp(vout.p, wv.data(), (size_t)n * vw * 4) == 0,
+                           "wide sort vw=%u n=%u", vw, n);
+                    gms_sort_destroy(sc);
+                }
+            }
+        }
+    }
+    printf("wide done\n");
     printf(failures ? "EMU CHECK FAILED (%d)\n" : "EMU CHECK OK\n", failures);
     return failures ? 1 : 0;
 }

@@ -40,6 +40,10 @@ struct alignas(16) uint4 {
     uint32_t x, y, z, w;
 };
 inline uint4 make_uint4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) { return uint4{a, b, c, d}; }
+struct alignas(8) uint2 {
+    uint32_t x, y;
+};
+inline uint2 make_uint2(uint32_t a, uint32_t b) { return uint2{a, b}; }
 
 namespace gms_emu {
 extern thread_local gms_emu_dim3 thread_idx;
@@ -64,6 +68,9 @@ inline uint32_t atomicMax(uint32_t* p, uint32_t v) {
     while (old < v && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
     }
     return old;
+}
+inline uint64_t __umul64hi(uint64_t a, uint64_t b) {
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
 }
 inline uint32_t __umulhi(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
 inline float __uint_as_float(uint32_t u) {

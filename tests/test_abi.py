@@ -1,6 +1,7 @@
 """C-ABI library checks that need no GPU: the library loads, exports exactly the
 symbols include/gpumultisplit.h declares, and its host-side logic (bucket-id
 arithmetic, tile policy, argument validation before any HIP call) is right."""
+import ctypes
 import os
 import re
 import subprocess
@@ -145,3 +146,97 @@ def test_library_has_no_undefined_library_symbols():
                        text=True, check=True)
     missing = [ln for ln in r.stdout.splitlines() if "gms::" in ln or " gms_" in ln]
     assert not missing, missing
+
+
+# 64-bit identifiers: the host entry point runs the kernels' AnyF64 code.
+def _py_bucket64(kind, B, param, width, k):
+    if kind == gms.BUCKET_RANGE:
+        b = (k * B) >> 64
+    elif kind == gms.BUCKET_DELTA:
+        b = k // param
+    elif kind == gms.BUCKET_DIGIT:
+        b = (k >> param) & (((1 << width) - 1) if width else B - 1)
+    elif kind == gms.BUCKET_IDENTITY:
+        b = k
+    else:
+        b = k & ((1 << param) - 1)
+    return min(b, B - 1)
+
+
+def _ids64(B, kind, param, width, keys):
+    L = gms.load_library()
+    k = np.ascontiguousarray(keys, dtype=np.uint64)
+    ids = np.zeros(max(k.size, 1), np.uint32)
+    st = L.gms_bucket_ids64_host(B, gms.gms_bucket_desc64(kind, width, param), k.ctypes.data,
+                                 ids.ctypes.data, k.size)
+    assert st == 0
+    return ids[:k.size]
+
+
+@pytest.mark.parametrize("delta", [1, 2, 3, 7, 1000, (1 << 32) - 1, 1 << 32, (1 << 32) + 1,
+                                   (1 << 56) + 12345, (1 << 63) - 1, 1 << 63, (1 << 63) + 1,
+                                   (1 << 64) - 2, (1 << 64) - 1, 6700417 * 641, 0x9E3779B97F4A7C15])
+def test_delta64_magic_division_is_exact(delta):
+    g = W.rng(delta % 997 + 3)
+    keys = [0, 1, (1 << 64) - 1, (1 << 64) - 2, 1 << 63, (1 << 63) - 1]
+    top = ((1 << 64) - 1) // delta
+    for m in list(range(0, 40)) + [int(x) for x in g.integers(0, top + 1, 40, dtype=np.uint64)] + [top]:
+        for o in (-1, 0, 1):
+            k = m * delta + o
+            if 0 <= k < (1 << 64):
+                keys.append(k)
+    keys += [int(x) for x in g.integers(0, 1 << 64, 200, dtype=np.uint64, endpoint=False)]
+    got = _ids64(256, gms.BUCKET_DELTA, delta, 0, keys)
+    want = [min(k // delta, 255) for k in keys]
+    np.testing.assert_array_equal(got, np.array(want, dtype=np.uint32))
+    # unclamped quotient check through a 1-bucket-per-quotient view: B = 256 and
+    # keys whose quotient is below 256
+    small = [q * delta + r for q in range(0, 256, 17) for r in (0, delta - 1) if q * delta + r < (1 << 64)]
+    np.testing.assert_array_equal(_ids64(256, gms.BUCKET_DELTA, delta, 0, small),
+                                  np.array([min(k // delta, 255) for k in small], dtype=np.uint32))
+
+
+@pytest.mark.parametrize("B", [2, 3, 100, 255, 256])
+def test_range64_boundaries(B):
+    bnd = [(j << 64) // B for j in range(B)] + [-(-(j << 64) // B) for j in range(1, B)]
+    keys = sorted({v + d for v in bnd for d in (-1, 0, 1) if 0 <= v + d < (1 << 64)} | {(1 << 64) - 1})
+    np.testing.assert_array_equal(_ids64(B, gms.BUCKET_RANGE, 0, 0, keys),
+                                  np.array([_py_bucket64(gms.BUCKET_RANGE, B, 0, 0, k) for k in keys],
+                                           dtype=np.uint32))
+
+
+@pytest.mark.parametrize("kind,B,param,width", [
+    (gms.BUCKET_DIGIT, 256, 56, 8), (gms.BUCKET_DIGIT, 16, 60, 4), (gms.BUCKET_DIGIT, 64, 3, 0),
+    (gms.BUCKET_DIGIT, 2, 63, 1), (gms.BUCKET_IDENTITY, 200, 0, 0), (gms.BUCKET_LOWBITS, 32, 5, 0),
+])
+def test_other_identifiers64(kind, B, param, width):
+    g = W.rng(B + param)
+    keys = [0, 1, (1 << 64) - 1, 1 << 63, 199, 200, 255, 256] + \
+        [int(x) for x in g.integers(0, 1 << 64, 300, dtype=np.uint64, endpoint=False)]
+    np.testing.assert_array_equal(_ids64(B, kind, param, width, keys),
+                                  np.array([_py_bucket64(kind, B, param, width, k) for k in keys],
+                                           dtype=np.uint32))
+
+
+def test_wide_entry_points_validate_before_any_hip_call():
+    L = gms.load_library()
+    h = ctypes.c_void_p()
+    assert L.gms_multisplit_create(256, ctypes.byref(h)) == 0
+    d = gms.gms_bucket_desc64(gms.BUCKET_RANGE, 0, 0)
+    # not allocated, bad widths, shift past the key, delta 0
+    assert L.gms_multisplit_wide(h, 8, None, 16, None, 1, 8, 0, d, None, None, None) == 4
+    assert L.gms_multisplit_wide(h, 8, None, 16, None, 1, 6, 0, d, None, None, None) == 1
+    assert L.gms_multisplit_wide(h, 8, None, 16, None, 1, 8, 2, d, None, None, None) == 1
+    assert L.gms_multisplit_wide(h, 8, 24, 16, None, 1, 8, 0, d, None, None, None) == 1
+    assert L.gms_multisplit_destroy(h) == 0
+    assert L.gms_bucket_ids64_host(256, gms.gms_bucket_desc64(gms.BUCKET_DIGIT, 8, 64), None,
+                                   None, 0) == 1
+    assert L.gms_bucket_ids64_host(256, gms.gms_bucket_desc64(gms.BUCKET_DELTA, 0, 0), None,
+                                   None, 0) == 1
+    assert L.gms_bucket_ids64_host(257, d, None, None, 0) == 2
+    s = ctypes.c_void_p()
+    assert L.gms_sort_create(ctypes.byref(s)) == 0
+    assert L.gms_sort_wide(s, 8, 16, None, None, 1, 8, 0, None) == 4
+    assert L.gms_sort_allocate_wide(s, 10, 3, 8, 0) == 1  # digit width outside 4..8
+    assert L.gms_sort_allocate_wide(s, 10, 8, 8, 2) == 1
+    assert L.gms_sort_destroy(s) == 0

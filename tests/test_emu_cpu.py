@@ -345,3 +345,83 @@ def test_emulated_histogram_range_edge_sets(emu):
         want = oracle.histogram_range(x, lv)
         assert want.sum() > lv.size // 2
         np.testing.assert_array_equal(h, want, err_msg=str(lv[:8]))
+
+
+# ------------------------------------------------- 64-bit keys and values
+WIDE_CASES = [  # (B, kind, param, width)
+    (256, gms.BUCKET_RANGE, 0, 0), (2, gms.BUCKET_RANGE, 0, 0), (100, gms.BUCKET_DELTA, (1 << 57) + 5, 0),
+    (64, gms.BUCKET_DIGIT, 58, 6), (17, gms.BUCKET_IDENTITY, 0, 0), (8, gms.BUCKET_LOWBITS, 3, 0),
+    (1, gms.BUCKET_RANGE, 0, 0),
+]
+
+
+def _wide_keys(n, seed, kbytes):
+    g = np.random.default_rng(seed)
+    if kbytes == 8:
+        k = g.integers(0, 1 << 64, n, dtype=np.uint64, endpoint=False)
+        k[::5] &= np.uint64(0xFF)  # small keys for identity / low-bit identifiers
+        return k
+    return g.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+
+
+def emu_multisplit_wide(L, keys, vals, B, kind, param, width):
+    n = keys.size
+    kb = keys.dtype.itemsize
+    vb = 0 if vals is None else vals.dtype.itemsize
+    h = ctypes.c_void_p()
+    assert L.gms_multisplit_create(B, ctypes.byref(h)) == 0
+    try:
+        assert L.gms_multisplit_allocate(h, n, 1) == 0
+        kout = np.full(max(n, 1), 0x5A, keys.dtype)
+        vout = None if vals is None else np.full(max(n, 1), 0x5A, vals.dtype)
+        offs = np.zeros(B, np.uint32)
+        st = L.gms_multisplit_wide(h, p(keys), p(vals), p(kout), p(vout), n, kb, vb,
+                                   gms.gms_bucket_desc64(kind, width, param), None, p(offs), None)
+        assert st == 0, st
+        return kout[:n], (None if vals is None else vout[:n]), offs
+    finally:
+        L.gms_multisplit_destroy(h)
+
+
+@pytest.mark.parametrize("case", range(len(WIDE_CASES)))
+@pytest.mark.parametrize("kbytes,vbytes", [(8, 0), (8, 4), (8, 8), (4, 8)])
+def test_emulated_multisplit_wide_matches_oracle(emu, case, kbytes, vbytes):
+    B, kind, param, width = WIDE_CASES[case]
+    if kbytes == 4 and param > 0xFFFFFFFF:
+        param = 1000
+    if kbytes == 4 and kind == gms.BUCKET_DIGIT:
+        param = 26
+    tile = 256 * (16 if kbytes + vbytes <= 8 else 12)
+    for n in (0, 1, 77, tile, 2 * tile + 5):
+        keys = _wide_keys(n, seed=n + case, kbytes=kbytes)
+        vals = None
+        if vbytes:
+            vals = (np.arange(n, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)).astype(
+                np.uint64 if vbytes == 8 else np.uint32)
+        got = emu_multisplit_wide(emu, keys, vals, B, kind, param, width)
+        wk, wv, wo = oracle.multisplit_wide(keys, B, kind, param, vals, width)
+        np.testing.assert_array_equal(got[0], wk)
+        np.testing.assert_array_equal(got[2], wo)
+        if vbytes:
+            np.testing.assert_array_equal(got[1], wv)
+
+
+@pytest.mark.parametrize("kbytes,vbytes", [(8, 0), (8, 8), (8, 4), (4, 8)])
+def test_emulated_sort_wide(emu, kbytes, vbytes):
+    n = 2 * 256 * 12 + 9
+    keys = _wide_keys(n, seed=3, kbytes=kbytes)
+    keys[::11] = keys[0]
+    vals = None if not vbytes else np.arange(n, dtype=np.uint64 if vbytes == 8 else np.uint32)
+    h = ctypes.c_void_p()
+    assert emu.gms_sort_create(ctypes.byref(h)) == 0
+    try:
+        assert emu.gms_sort_allocate_wide(h, n, 8, kbytes, vbytes) == 0
+        kout = np.zeros(n, keys.dtype)
+        vout = None if vals is None else np.zeros(n, vals.dtype)
+        assert emu.gms_sort_wide(h, p(keys), p(kout), p(vals), p(vout), n, kbytes, vbytes, None) == 0
+    finally:
+        emu.gms_sort_destroy(h)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(kout, keys[order])
+    if vbytes:
+        np.testing.assert_array_equal(vout, vals[order])

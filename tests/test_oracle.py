@@ -252,3 +252,92 @@ def test_oracle_histogram_range_against_searchsorted(nbins):
     k = W.uniform_keys(20000, seed=nbins + 1)
     k[:nbins + 1] = lu
     np.testing.assert_array_equal(oracle.histogram_range(k, lu), _np_histogram_range(k, lu))
+
+
+# --------------------------------------------------------------- 64-bit keys
+def _py_bucket64(kind, B, param, width, k):
+    if kind == oracle.RANGE:
+        b = (k * B) >> 64
+    elif kind == oracle.DELTA:
+        b = k // param
+    elif kind == oracle.DIGIT:
+        b = (k >> param) & (((1 << width) - 1) if width else B - 1)
+    elif kind == oracle.IDENTITY:
+        b = k
+    else:
+        b = k & ((1 << param) - 1)
+    return min(b, B - 1)
+
+
+WIDE_KINDS = [  # (kind, B, param, width)
+    (oracle.RANGE, 256, 0, 0), (oracle.RANGE, 3, 0, 0), (oracle.RANGE, 2, 0, 0),
+    (oracle.DELTA, 100, (1 << 57) + 12345, 0), (oracle.DELTA, 256, 3, 0),
+    (oracle.DIGIT, 256, 56, 8), (oracle.DIGIT, 16, 31, 4), (oracle.DIGIT, 64, 0, 0),
+    (oracle.IDENTITY, 200, 0, 0), (oracle.LOWBITS, 32, 5, 0), (oracle.LOWBITS, 256, 40, 0),
+]
+
+
+def _wide_keys(n, seed):
+    g = np.random.default_rng(seed)
+    k = g.integers(0, 1 << 64, n, dtype=np.uint64, endpoint=False)
+    edge = np.array([0, 1, (1 << 64) - 1, (1 << 63), (1 << 63) - 1, (1 << 32), (1 << 32) - 1,
+                     199, 200, 201], dtype=np.uint64)
+    return np.concatenate([edge, k, k[:50] & np.uint64(0xFF)]).astype(np.uint64)
+
+
+@pytest.mark.parametrize("case", range(len(WIDE_KINDS)))
+def test_oracle_bucket_ids64_against_python_integers(case):
+    kind, B, param, width = WIDE_KINDS[case]
+    keys = _wide_keys(500, case)
+    if kind == oracle.RANGE:  # every range boundary j * 2^64 / B and its neighbours
+        bnd = [(j << 64) // B for j in range(B)] + [-(-(j << 64) // B) for j in range(1, B)]
+        extra = [v + d for v in bnd for d in (-1, 0, 1) if 0 <= v + d < (1 << 64)]
+        keys = np.concatenate([keys, np.array(extra, dtype=np.uint64)])
+    if kind == oracle.DELTA:
+        extra = [j * param + d for j in range(1, 8) for d in (-1, 0, 1) if j * param + d < (1 << 64)]
+        keys = np.concatenate([keys, np.array(extra, dtype=np.uint64)])
+    got = oracle.bucket_ids64(keys, B, kind, param, width)
+    want = [_py_bucket64(kind, B, param, width, int(k)) for k in keys]
+    np.testing.assert_array_equal(got, np.array(want, dtype=np.uint32))
+
+
+@pytest.mark.parametrize("case", range(len(WIDE_KINDS)))
+@pytest.mark.parametrize("vbits", [0, 32, 64])
+def test_oracle_multisplit_wide_against_stable_argsort(case, vbits):
+    kind, B, param, width = WIDE_KINDS[case]
+    keys = _wide_keys(3000, 100 + case)
+    ids = np.array([_py_bucket64(kind, B, param, width, int(k)) for k in keys], dtype=np.int64)
+    order = np.argsort(ids, kind="stable")
+    vals = None
+    if vbits:
+        vt = np.uint64 if vbits == 64 else np.uint32
+        vals = (np.arange(keys.size, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)).astype(vt)
+    ko, vo, offs = oracle.multisplit_wide(keys, B, kind, param, vals, width)
+    np.testing.assert_array_equal(ko, keys[order])
+    np.testing.assert_array_equal(offs, np.concatenate([[0], np.cumsum(np.bincount(ids, minlength=B))[:-1]]))
+    if vbits:
+        np.testing.assert_array_equal(vo, vals[order])
+
+
+def test_oracle_multisplit_wide_u32_keys_match_32bit_oracle():
+    # 32-bit keys with 64-bit values: the bucket ids equal the 32-bit oracle's
+    keys = W.uniform_keys(5000, seed=9)
+    vals = np.arange(5000, dtype=np.uint64) << np.uint64(20)
+    ko, vo, offs = oracle.multisplit_wide(keys, 100, oracle.RANGE, 0, vals)
+    wk, wv, wo = oracle.multisplit(keys, 100, oracle.RANGE, 0, np.arange(5000, dtype=np.uint32))
+    np.testing.assert_array_equal(ko, wk)
+    np.testing.assert_array_equal(vo, vals[wv])
+    np.testing.assert_array_equal(offs, wo)
+
+
+@pytest.mark.parametrize("digit_bits", [8, 7, 5])
+@pytest.mark.parametrize("vbits", [0, 32, 64])
+def test_oracle_sort_wide_against_numpy(digit_bits, vbits):
+    keys = _wide_keys(4000, digit_bits)
+    keys[::7] = keys[0]  # duplicates expose instability
+    order = np.argsort(keys, kind="stable")
+    vals = None if not vbits else np.arange(keys.size, dtype=np.uint64 if vbits == 64 else np.uint32)
+    ko, vo = oracle.sort_wide(keys, digit_bits, vals)
+    np.testing.assert_array_equal(ko, keys[order])
+    if vbits:
+        np.testing.assert_array_equal(vo, vals[order])
