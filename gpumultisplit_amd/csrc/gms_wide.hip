@@ -59,27 +59,36 @@ __device__ __forceinline__ void load_elems(const uint32_t* __restrict__ p, size_
     }
 }
 
+// Prescan: the tile's key words are read as one flat array in 16-byte stripes
+// (the 32-bit prescan's lane-contiguous loads); word 4*(j*256+t)+i belongs to
+// element (4*(j*256+t)+i) / KW, and a 64-bit key never straddles a stripe.
 template <int LOG_B, int K, int KW, bool VEC, class F>
 __global__ __launch_bounds__(kThreads) void ms_wide_prescan_kernel(const uint32_t* __restrict__ kin,
                                                                    uint32_t n, uint32_t num_tiles,
                                                                    uint32_t* __restrict__ H, F f) {
     constexpr int B = 1 << LOG_B;
+    constexpr int WPT = K * KW;  // key words per thread
     constexpr uint32_t TILE = kThreads * K;
+    static_assert(WPT % 4 == 0, "16-byte stripes");
     __shared__ __attribute__((aligned(16))) uint32_t s_h[kWaves * B];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
     const uint32_t t = xcd_tile(blockIdx.x, num_tiles);
     const uint32_t nvalid = tile_valid(n, t, TILE);
-    const size_t base = (size_t)t * TILE;
-    uint32_t x[K * KW];
-    if (nvalid == TILE) load_elems<KW, K, VEC, true>(kin, base, TILE, x);
-    else load_elems<KW, K, false, false>(kin, base, nvalid, x);
+    const size_t base = (size_t)t * TILE * KW;
+    uint32_t x[WPT];
+    if (nvalid == TILE) load_striped4_full<WPT, VEC>(kin, base, x);
+    else load_striped4_guarded<WPT>(kin, base, nvalid * KW, x);
     uint32_t* my = s_h + wid * B;
     for (uint32_t i = lane; i < (uint32_t)B; i += 64) my[i] = 0;
     GMS_WAVE_SYNC();
-    const uint32_t e0 = tid * (uint32_t)K;
 #pragma unroll
-    for (int k = 0; k < K; ++k)
-        if (e0 + (uint32_t)k < nvalid) atomicAdd(&my[f(key_word_of<KW>(x, k))], 1u);
+    for (int j = 0; j < WPT / 4; ++j) {
+        const uint32_t e4 = (4u * ((uint32_t)j * kThreads + tid)) / KW;  // first element of the stripe
+#pragma unroll
+        for (int q = 0; q < 4 / KW; ++q)
+            if (e4 + (uint32_t)q < nvalid)
+                atomicAdd(&my[f(key_word_of<KW>(x + 4 * j, q))], 1u);
+    }
     __syncthreads();
     for (uint32_t b = tid; b < (uint32_t)B; b += kThreads) {
         uint32_t s = 0;

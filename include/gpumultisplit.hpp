@@ -20,14 +20,19 @@
 //    specialising gms::bucket_traits<T> with
 //      static gms_bucket_desc desc(const T&, uint32_t num_buckets);
 //    mapping it to one of the built-in kinds.
-//  * KeyT and ValueT must be 32-bit (the reference stages keys and values as
-//    uint32_t in shared memory, e.g. bms_postscan_pairs.cuh:1000).
+//  * KeyT and ValueT of multisplits and sorts may be 32- or 64-bit (any 8-byte
+//    operand takes the 64-bit kernels, up to 256 buckets); compaction and
+//    histograms take 32-bit keys.  A 64-bit identifier parameter (a delta
+//    above 2^32) needs gms::delta_func64.  Sorts of 64-bit keys or values are
+//    allocated through the typed forms, e.g.
+//    multisplit_sort_8bit_allocate_pairs<uint64_t, uint64_t>(n, ctx).
 #ifndef GPUMULTISPLIT_HPP_
 #define GPUMULTISPLIT_HPP_
 
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 
 #include "gpumultisplit.h"
 
@@ -64,7 +69,15 @@ struct test_bucket {
     uint32_t operator()(key_type a) const { return a & ((1u << flag) - 1); }
 };
 
-// floor(key * B / 2^32): B equal key ranges.
+// key / delta_ with a 64-bit delta (for 64-bit keys).
+struct delta_func64 {
+    delta_func64(uint64_t delta) : delta_(delta) {}
+    uint64_t delta_;
+    uint32_t operator()(uint64_t a) const { return (uint32_t)(a / delta_); }
+};
+
+// floor(key * B / 2^32) on 32-bit keys, floor(key * B / 2^64) on 64-bit ones:
+// B equal key ranges.
 struct range_bucket {
     explicit range_bucket(uint32_t num_buckets) : num_buckets_(num_buckets) {}
     uint32_t num_buckets_;
@@ -111,10 +124,47 @@ inline gms_bucket_desc desc_of(const F& f, uint32_t num_buckets) {
     return bucket_traits<typename std::decay<F>::type>::desc(f, num_buckets);
 }
 
+// The 64-bit descriptor: bucket_traits<T>::desc64 where a functor has one,
+// else its 32-bit descriptor widened.
+template <class T, class = void>
+struct has_desc64 : std::false_type {};
+template <class T>
+struct has_desc64<T, decltype((void)bucket_traits<T>::desc64(std::declval<const T&>(), 0u))>
+    : std::true_type {};
+
+template <class F>
+inline gms_bucket_desc64 desc64_of(const F& f, uint32_t num_buckets) {
+    using T = typename std::decay<F>::type;
+    if constexpr (has_desc64<T>::value) {
+        return bucket_traits<T>::desc64(f, num_buckets);
+    } else {
+        const gms_bucket_desc d = bucket_traits<T>::desc(f, num_buckets);
+        return gms_bucket_desc64{d.kind, d.width, d.param};
+    }
+}
+
+template <>
+struct bucket_traits<delta_func64> {
+    static gms_bucket_desc desc(const delta_func64& f, uint32_t) {
+        if (f.delta_ > 0xFFFFFFFFull)
+            throw error(GMS_ERR_INVALID_ARG, "delta_func64: delta needs 64-bit keys");
+        return gms_bucket_desc{GMS_BUCKET_DELTA, (uint32_t)f.delta_, 0};
+    }
+    static gms_bucket_desc64 desc64(const delta_func64& f, uint32_t) {
+        return gms_bucket_desc64{GMS_BUCKET_DELTA, 0, f.delta_};
+    }
+};
+
 template <class T>
 inline void require_32bit(const char* what) {
     static_assert(sizeof(T) == 4, "keys and values must be 32-bit");
     (void)what;
+}
+
+template <class T>
+constexpr uint32_t elem_bytes() {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "keys and values must be 32- or 64-bit");
+    return (uint32_t)sizeof(T);
 }
 
 }  // namespace gms
@@ -184,12 +234,19 @@ template <typename KeyT, typename bucket_t>
 inline void multisplit_key_only(KeyT* d_key_in, KeyT* d_key_out, uint32_t num_elements,
                                 multisplit_context& context, bucket_t bucket_identifier,
                                 uint32_t* h_offsets = nullptr, void* stream = nullptr) {
-    gms::require_32bit<KeyT>("key");
-    gms::check(gms_multisplit_key_only(context.handle, reinterpret_cast<const uint32_t*>(d_key_in),
-                                       reinterpret_cast<uint32_t*>(d_key_out), num_elements,
-                                       gms::desc_of(bucket_identifier, context.num_buckets),
-                                       h_offsets, stream),
-               "multisplit_key_only");
+    if constexpr (gms::elem_bytes<KeyT>() == 8) {
+        gms::check(gms_multisplit_wide(context.handle, d_key_in, nullptr, d_key_out, nullptr,
+                                       num_elements, 8, 0,
+                                       gms::desc64_of(bucket_identifier, context.num_buckets),
+                                       nullptr, h_offsets, stream),
+                   "multisplit_key_only");
+    } else {
+        gms::check(gms_multisplit_key_only(context.handle, reinterpret_cast<const uint32_t*>(d_key_in),
+                                           reinterpret_cast<uint32_t*>(d_key_out), num_elements,
+                                           gms::desc_of(bucket_identifier, context.num_buckets),
+                                           h_offsets, stream),
+                   "multisplit_key_only");
+    }
 }
 
 // src/api/multisplit.cuh:199-313.
@@ -198,15 +255,22 @@ inline void multisplit_key_value(KeyT* d_key_in, ValueT* d_value_in, KeyT* d_key
                                  ValueT* d_value_out, uint32_t num_elements,
                                  multisplit_context& context, bucket_t bucket_identifier,
                                  uint32_t* h_offsets = nullptr, void* stream = nullptr) {
-    gms::require_32bit<KeyT>("key");
-    gms::require_32bit<ValueT>("value");
-    gms::check(gms_multisplit_key_value(
-                   context.handle, reinterpret_cast<const uint32_t*>(d_key_in),
-                   reinterpret_cast<const uint32_t*>(d_value_in),
-                   reinterpret_cast<uint32_t*>(d_key_out), reinterpret_cast<uint32_t*>(d_value_out),
-                   num_elements, gms::desc_of(bucket_identifier, context.num_buckets), h_offsets,
-                   stream),
-               "multisplit_key_value");
+    constexpr uint32_t kb = gms::elem_bytes<KeyT>(), vb = gms::elem_bytes<ValueT>();
+    if constexpr (kb == 8 || vb == 8) {
+        gms::check(gms_multisplit_wide(context.handle, d_key_in, d_value_in, d_key_out, d_value_out,
+                                       num_elements, kb, vb,
+                                       gms::desc64_of(bucket_identifier, context.num_buckets),
+                                       nullptr, h_offsets, stream),
+                   "multisplit_key_value");
+    } else {
+        gms::check(gms_multisplit_key_value(
+                       context.handle, reinterpret_cast<const uint32_t*>(d_key_in),
+                       reinterpret_cast<const uint32_t*>(d_value_in),
+                       reinterpret_cast<uint32_t*>(d_key_out), reinterpret_cast<uint32_t*>(d_value_out),
+                       num_elements, gms::desc_of(bucket_identifier, context.num_buckets), h_offsets,
+                       stream),
+                   "multisplit_key_value");
+    }
 }
 
 // subproblem_size_bms_key_only / _key_value (src/api/bms_api.h:23-89).
@@ -237,28 +301,46 @@ inline void multisplit_sort_release_memory(ms_sort_context& context) {
 // multisplit_sort_Nbit (:320-559), multisplit_sort_Nbit_pairs
 // (multisplit_sort_pairs.cuh:21-260): keys in, keys out, values in, values out.
 #define GMS_SORT_API(N)                                                                          \
+    template <typename KeyT = uint32_t>                                                          \
     inline void multisplit_sort_##N##bit_allocate(uint32_t n_elements, ms_sort_context& ctx) {  \
-        gms::check(gms_sort_allocate(ctx.handle, n_elements, N, 0),                              \
+        constexpr uint32_t kb = gms::elem_bytes<KeyT>();                                         \
+        gms::check(kb == 4 ? gms_sort_allocate(ctx.handle, n_elements, N, 0)                     \
+                           : gms_sort_allocate_wide(ctx.handle, n_elements, N, kb, 0),           \
                    "multisplit_sort_" #N "bit_allocate");                                        \
     }                                                                                            \
+    template <typename KeyT = uint32_t, typename ValueT = uint32_t>                              \
     inline void multisplit_sort_##N##bit_allocate_pairs(uint32_t n_elements,                     \
                                                         ms_sort_context& ctx) {                  \
-        gms::check(gms_sort_allocate(ctx.handle, n_elements, N, 1),                              \
+        constexpr uint32_t kb = gms::elem_bytes<KeyT>(), vb = gms::elem_bytes<ValueT>();         \
+        gms::check(kb == 4 && vb == 4 ? gms_sort_allocate(ctx.handle, n_elements, N, 1)          \
+                                      : gms_sort_allocate_wide(ctx.handle, n_elements, N, kb, vb), \
                    "multisplit_sort_" #N "bit_allocate_pairs");                                  \
     }                                                                                            \
-    inline void multisplit_sort_##N##bit(uint32_t* d_key_in, uint32_t* d_key_out,                \
-                                         uint32_t n_elements, ms_sort_context& ctx,              \
-                                         void* stream = nullptr) {                               \
-        gms::check(gms_sort(ctx.handle, d_key_in, d_key_out, nullptr, nullptr, n_elements, N,    \
-                            stream),                                                             \
+    template <typename KeyT>                                                                     \
+    inline void multisplit_sort_##N##bit(KeyT* d_key_in, KeyT* d_key_out, uint32_t n_elements,   \
+                                         ms_sort_context& ctx, void* stream = nullptr) {         \
+        constexpr uint32_t kb = gms::elem_bytes<KeyT>();                                         \
+        gms::check(kb == 4 ? gms_sort(ctx.handle, reinterpret_cast<const uint32_t*>(d_key_in),   \
+                                      reinterpret_cast<uint32_t*>(d_key_out), nullptr, nullptr,  \
+                                      n_elements, N, stream)                                     \
+                           : gms_sort_wide(ctx.handle, d_key_in, d_key_out, nullptr, nullptr,    \
+                                           n_elements, kb, 0, stream),                           \
                    "multisplit_sort_" #N "bit");                                                 \
     }                                                                                            \
-    inline void multisplit_sort_##N##bit_pairs(uint32_t* d_key_in, uint32_t* d_key_out,          \
-                                               uint32_t* d_value_in, uint32_t* d_value_out,      \
+    template <typename KeyT, typename ValueT>                                                    \
+    inline void multisplit_sort_##N##bit_pairs(KeyT* d_key_in, KeyT* d_key_out,                  \
+                                               ValueT* d_value_in, ValueT* d_value_out,          \
                                                uint32_t n_elements, ms_sort_context& ctx,        \
                                                void* stream = nullptr) {                         \
-        gms::check(gms_sort(ctx.handle, d_key_in, d_key_out, d_value_in, d_value_out,            \
-                            n_elements, N, stream),                                              \
+        constexpr uint32_t kb = gms::elem_bytes<KeyT>(), vb = gms::elem_bytes<ValueT>();         \
+        gms::check(kb == 4 && vb == 4                                                            \
+                       ? gms_sort(ctx.handle, reinterpret_cast<const uint32_t*>(d_key_in),       \
+                                  reinterpret_cast<uint32_t*>(d_key_out),                        \
+                                  reinterpret_cast<const uint32_t*>(d_value_in),                 \
+                                  reinterpret_cast<uint32_t*>(d_value_out), n_elements, N,       \
+                                  stream)                                                        \
+                       : gms_sort_wide(ctx.handle, d_key_in, d_key_out, d_value_in, d_value_out, \
+                                       n_elements, kb, vb, stream),                              \
                    "multisplit_sort_" #N "bit_pairs");                                           \
     }
 GMS_SORT_API(4)

@@ -27,6 +27,10 @@ extern "C" {
 int ms_oracle_multisplit(uint32_t kind, uint32_t B, uint32_t param, const uint32_t* kin,
                          const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint64_t n,
                          uint32_t* offsets);
+int ms_oracle_multisplit_wide(uint32_t kind, uint32_t B, uint64_t param, uint32_t width,
+                              uint32_t key_words, uint32_t value_words, const uint32_t* kin,
+                              const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint64_t n,
+                              uint32_t* offsets);
 uint64_t ms_oracle_compaction(uint32_t kind, uint32_t param, const uint32_t* kin,
                               const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint64_t n);
 }
@@ -185,6 +189,56 @@ static void compaction_case(uint32_t n) {
     compaction_release_memory(ctx);
 }
 
+// multisplit_key_value<uint64_t, uint64_t> with a 64-bit delta, and the
+// typed sort of 64-bit keys carrying 32-bit values.
+static void wide_cases(uint32_t n) {
+    const uint32_t B = 200;
+    const uint64_t delta = 0xFFFFFFFFFFFFFFFFull / 150;
+    std::vector<uint32_t> k(2 * n), v(2 * n), v32(n);
+    for (uint32_t i = 0; i < 2 * n; ++i) k[i] = rnd();
+    for (uint32_t i = 0; i < n; ++i) {
+        v[2 * i] = i;
+        v[2 * i + 1] = ~i;
+        v32[i] = i;
+    }
+    DevBuf dk(2 * n), dv(2 * n), dko(2 * n), dvo(2 * n);
+    dk.put(k);
+    dv.put(v);
+    multisplit_context ctx(B);
+    multisplit_allocate_key_value(n, ctx);
+    std::vector<uint32_t> offs(B, 0xFFFFFFFFu);
+    multisplit_key_value(reinterpret_cast<uint64_t*>(dk.p), reinterpret_cast<uint64_t*>(dv.p),
+                         reinterpret_cast<uint64_t*>(dko.p), reinterpret_cast<uint64_t*>(dvo.p), n,
+                         ctx, gms::delta_func64(delta), offs.data());
+    std::vector<uint32_t> wk(2 * n), wv(2 * n), wo(B);
+    ms_oracle_multisplit_wide(1, B, delta, 0, 2, 2, k.data(), v.data(), wk.data(), wv.data(), n,
+                              wo.data());
+    EXPECT(dko.get(2 * n) == wk, "multisplit_key_value<u64,u64> keys n=%u", n);
+    EXPECT(dvo.get(2 * n) == wv, "multisplit_key_value<u64,u64> values n=%u", n);
+    EXPECT(offs == wo, "multisplit_key_value<u64,u64> h_offsets n=%u", n);
+    multisplit_release_memory(ctx);
+
+    DevBuf dv32(n), dvo32(n);
+    dv32.put(v32);
+    ms_sort_context sc;
+    multisplit_sort_8bit_allocate_pairs<uint64_t, uint32_t>(n, sc);
+    multisplit_sort_8bit_pairs(reinterpret_cast<uint64_t*>(dk.p), reinterpret_cast<uint64_t*>(dko.p),
+                               dv32.p, dvo32.p, n, sc);
+    std::vector<uint64_t> k64(n);
+    for (uint32_t i = 0; i < n; ++i) k64[i] = (uint64_t)k[2 * i] | ((uint64_t)k[2 * i + 1] << 32);
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return k64[a] < k64[b]; });
+    std::vector<uint32_t> sk(2 * n);
+    for (uint32_t i = 0; i < n; ++i) {
+        sk[2 * i] = (uint32_t)k64[order[i]];
+        sk[2 * i + 1] = (uint32_t)(k64[order[i]] >> 32);
+    }
+    EXPECT(dko.get(2 * n) == sk, "multisplit_sort_8bit_pairs<u64,u32> keys n=%u", n);
+    EXPECT(dvo32.get(n) == order, "multisplit_sort_8bit_pairs<u64,u32> values n=%u", n);
+    multisplit_sort_release_memory(sc);
+}
+
 static void error_cases() {
     multisplit_context ctx(256);
     bool threw = false;
@@ -210,6 +264,7 @@ int main(int argc, char** argv) {
     for (int bits : {7, 8})
         for (bool pairs : {false, true}) sort_case(n, bits, pairs);
     compaction_case(n);
+    wide_cases(n);
     error_cases();
     if (failures) {
         printf("API DRIVER: %d failure(s)\n", failures);
