@@ -111,6 +111,15 @@ static int class_of(uint32_t num_buckets) {
     return l < 1 ? 1 : l;
 }
 
+gms_status BatchEngine::reserve_tables(uint32_t instances, uint64_t work_items) {
+    for (int k = 0; k < 2; ++k) {
+        gms_status st = h_inst_[k].reserve(instances);
+        if (st == GMS_SUCCESS) st = h_work_[k].reserve((size_t)work_items);
+        if (st != GMS_SUCCESS) return st;
+    }
+    return GMS_SUCCESS;
+}
+
 gms_status BatchEngine::plan(const gms_batch_item* items, uint32_t count, int slot) {
     const uint32_t K = tuning.items_per_thread;
     if (K != 8 && K != 16 && K != 24 && K != 32) return GMS_ERR_INVALID_ARG;

@@ -70,6 +70,9 @@ class BatchEngine {
     gms_status upload(const BatchDevice& d, hipStream_t stream);
     // Queue reset, prescans, one scan, postscans -- all kernels, no memsets.
     gms_status launch(const BatchDevice& d, hipStream_t stream);
+    // Grows both staging slots to hold `instances` instances and `work_items`
+    // work items, so that plans up to that size allocate no pinned memory.
+    gms_status reserve_tables(uint32_t instances, uint64_t work_items);
     void release();
 
     uint64_t keys_total = 0;

@@ -11,6 +11,7 @@
 // stream order carries each pass's output to the next.  The arithmetic of a
 // request is the same as in the plain path and does not depend on what else
 // shares its step.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -346,7 +347,17 @@ gms_status gms_server_create(const gms_server_config* cfg, gms_server** out) {
         return GMS_ERR_NO_DEVICE;
     }
     gms_status st = s->arena.init(cfg->scratch_budget_bytes);
+    // Staging tables for steps of up to min(max_step_keys, 2^30) keys in 4096
+    // requests, allocated here rather than on the first large step (pinned
+    // allocations take milliseconds); larger steps still grow them.
+    if (st == GMS_SUCCESS) {
+        const uint64_t keys = std::min<uint64_t>(cfg->max_step_keys ? cfg->max_step_keys : (1ull << 31),
+                                                 1ull << 30);
+        st = s->eng.reserve_tables(4096, keys / ((uint64_t)kThreads * K) + 4096);
+    }
     if (st != GMS_SUCCESS) {
+        s->eng.release();
+        s->arena.destroy();
         delete s;
         return st;
     }
