@@ -56,7 +56,8 @@ def bench_served(args, torch, gms, dev):
     64 GiB scratch budget.  Capacity is measured with all requests arriving
     at once (median of three runs after one untimed full-size run, whose time
     is reported as first_run_ms); then Poisson arrivals at 30/60/90 % of it,
-    reporting throughput and p50/p99 latency.  The same requests as back-to-back plain calls give
+    three runs each, reporting throughput and p50/p99 latency of the run
+    with the median p99.  The same requests as back-to-back plain calls give
     the comparison throughput."""
     import numpy as np
     from gpumultisplit_amd import served as S
@@ -83,13 +84,18 @@ def bench_served(args, torch, gms, dev):
                            "first_run_ms": round(wall_first * 1e3, 3)}
         rates = []
         for i, frac in enumerate((0.3, 0.6, 0.9)):
+            # three runs of the same arrival times; the one with the median p99
+            # is reported, with every run's p99 beside it
             arr = S.poisson_arrivals(len(recs), frac * cap, seed=100 + i)
-            lat, wall = S.run_open_loop(srv, recs, arr)
+            runs = sorted((S.run_open_loop(srv, recs, arr) for _ in range(3)),
+                          key=lambda r: S.percentile(r[0], 99))
+            lat, wall = runs[1]
             rates.append({"offered_req_per_s": round(frac * cap, 1),
                           "achieved_req_per_s": round(len(recs) / wall, 1),
                           "gkeys_per_s": round(total / wall / 1e9, 3),
                           "p50_us": round(S.percentile(lat, 50), 1),
-                          "p99_us": round(S.percentile(lat, 99), 1)})
+                          "p99_us": round(S.percentile(lat, 99), 1),
+                          "p99_us_runs": [round(S.percentile(r[0], 99), 1) for r in runs]})
         out["poisson"] = rates
         st = srv.stats()
         out["steps"] = st["steps"]
