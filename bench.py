@@ -36,7 +36,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--workload", default="ms_kv_256",
                    help="ms_{kv,ko}_<B>[_<dist>][_n<log2n>] | sort{7,8}_{kv,ko} | "
-                        "hist256 | histrange256 | compact{50,5} | copy | wide_{kv,ko}_256 | wide_sort8_kv")
+                        "hist256 | histrange256 | compact{50,5} | copy | wide_{kv,ko}_256 | wide_sort8_kv | "
+                        "variant_sweep")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-stage-events", action="store_true")
     p.add_argument("--cpu-reps", type=int, default=5)
@@ -355,6 +356,54 @@ def bench_aux(args, torch, gms, W, dev):
     print(json.dumps(out), flush=True)
 
 
+def bench_variants(args, torch, gms, dev):
+    """Block-wide (BMS) against wave-level (WMS) multisplit: one JSON line per
+    (buckets, key-only/key-value, n) with both variants' time per call, the
+    measurement the automatic variant choice is set from.  Uniform keys,
+    range identifier, values = index; --steps timed calls after --warmup."""
+    L = gms.lib()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    g = torch.Generator(device=dev)
+    g.manual_seed(4321)
+    nmax = 1 << 27
+    kin = torch.randint(-(1 << 31), 1 << 31, (nmax,), dtype=torch.int32, device=dev, generator=g)
+    vin = torch.arange(nmax, dtype=torch.int32, device=dev)
+    kout, vout = torch.empty_like(kin), torch.empty_like(kin)
+    for B in (2, 4, 8, 16, 32, 64):
+        ctx = gms.multisplit_context(B)
+        gms.multisplit_allocate_key_value(nmax, ctx)
+        desc = gms.range_bucket(B).desc()
+        for kv in (False, True):
+            for log2n in (16, 20, 24, 27):
+                n = 1 << log2n
+                res = {}
+                for name, var in (("bms", gms.VARIANT_BMS), ("wms", gms.VARIANT_WMS)):
+                    gms.set_variant(ctx, var)
+
+                    def step():
+                        st = L.gms_multisplit(ctx.handle, kin.data_ptr(),
+                                              vin.data_ptr() if kv else None, kout.data_ptr(),
+                                              vout.data_ptr() if kv else None, n, desc, None,
+                                              stream)
+                        if st != 0:
+                            raise gms.GmsError(st, "gms_multisplit")
+                    for _ in range(args.warmup):
+                        step()
+                    torch.cuda.synchronize()
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    for _ in range(args.steps):
+                        step()
+                    e1.record()
+                    torch.cuda.synchronize()
+                    res[name] = e0.elapsed_time(e1) / args.steps
+                print(json.dumps({"buckets": B, "key_value": kv, "n": n,
+                                  "bms_ms": round(res["bms"], 5), "wms_ms": round(res["wms"], 5),
+                                  "wms_over_bms": round(res["wms"] / res["bms"], 3)}), flush=True)
+        gms.set_variant(ctx, gms.VARIANT_AUTO)
+        gms.multisplit_release_memory(ctx)
+
+
 def traffic_from_profiles(workload):
     """HBM bytes per postscan launch from committed rocprofv3 PMC summaries."""
     path = os.path.join(PROFILES, f"traffic_{workload}.txt")
@@ -400,6 +449,9 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev).cuda_stream
     cfg = parse_workload(args.workload)
+    if args.workload == "variant_sweep":
+        bench_variants(args, torch, gms, dev)
+        return
     if cfg["kind"] != "ms":
         bench_aux(args, torch, gms, W, dev)
         return

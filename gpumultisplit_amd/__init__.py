@@ -37,6 +37,7 @@ STATUS_NAMES = {
 }
 
 BUCKET_RANGE, BUCKET_DELTA, BUCKET_DIGIT, BUCKET_IDENTITY, BUCKET_LOWBITS = 0, 1, 2, 3, 4
+VARIANT_AUTO, VARIANT_BMS, VARIANT_WMS = 0, 1, 2
 
 
 class GmsError(RuntimeError):
@@ -109,6 +110,8 @@ _SIGS = [
     ("gms_multisplit", _I, [_P, _P, _P, _P, _P, _U32, gms_bucket_desc, _P, _P]),
     ("gms_multisplit_wide", _I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, gms_bucket_desc64, _P, _P,
                                  _P]),
+    ("gms_multisplit_set_variant", _I, [_P, _I]),
+    ("gms_subproblem_size_wms", _U32, [_U32, _I, ctypes.POINTER(_U32)]),
     ("gms_subproblem_size", _U32, [_U32, _I]),
     ("gms_tile_keys", _U32, [_U32, _I, _U32]),
     ("gms_multisplit_set_profiling", _I, [_P, _I]),
@@ -348,6 +351,27 @@ def subproblem_size_bms_key_only(num_buckets: int) -> int:
 
 def subproblem_size_bms_key_value(num_buckets: int) -> int:
     return int(lib().gms_subproblem_size(num_buckets, 1))
+
+
+def subproblem_size_wms_key_only(num_buckets: int):
+    """(keys per wave subproblem, keys per workgroup) of the wave-level
+    variant (src/api/wms_api.h:25-53); (0, 0) outside 2..64 buckets."""
+    blk = _U32()
+    sub = lib().gms_subproblem_size_wms(int(num_buckets), 0, ctypes.byref(blk))
+    return int(sub), int(blk.value) if sub else 0
+
+
+def subproblem_size_wms_key_value(num_buckets: int):
+    """src/api/wms_api.h:55-83 (same shape as key-only here)."""
+    blk = _U32()
+    sub = lib().gms_subproblem_size_wms(int(num_buckets), 1, ctypes.byref(blk))
+    return int(sub), int(blk.value) if sub else 0
+
+
+def set_variant(context, variant: int) -> None:
+    """Block-wide (VARIANT_BMS), wave-level (VARIANT_WMS, up to 64 buckets)
+    or automatic (VARIANT_AUTO) kernels for the context's multisplits."""
+    _check(lib().gms_multisplit_set_variant(context.handle, int(variant)), "set_variant")
 
 
 def tile_keys(num_buckets: int, key_value: bool, num_elements: int) -> int:

@@ -14,6 +14,7 @@ struct gms_multisplit_ctx {
     uint32_t num_buckets = 0;
     uint32_t capacity = 0;
     int allocated = 0;
+    int variant = GMS_VARIANT_AUTO;  // block-wide or wave-level kernels
     Scratch s;
     LargeScratch large;  // more than 256 buckets only
     Profiler prof;
@@ -54,7 +55,11 @@ size_t matrix_elems(uint32_t n, uint32_t num_buckets) {
     const size_t t_kv = ((size_t)n + tile_keys(lb, true, 0) - 1) / tile_keys(lb, true, 0);
     // the shortest tile of the 64-bit kernels (gms_multisplit_wide, gms_sort_wide)
     const size_t t_w = ((size_t)n + wide_tile_elems(2, 2) - 1) / wide_tile_elems(2, 2);
-    return ((size_t)1 << lb) * std::max(std::max(t_ko, t_kv), t_w);
+    // one column per wave subproblem of the wave-level variant (up to 64 buckets)
+    const size_t t_wms = num_buckets <= kWmsMaxBuckets
+                             ? ((size_t)n + wms_subproblem_keys() - 1) / wms_subproblem_keys()
+                             : 0;
+    return ((size_t)1 << lb) * std::max(std::max(std::max(t_ko, t_kv), t_w), t_wms);
 }
 
 gms_status scratch_alloc(Scratch& s, size_t hist_elems) {
@@ -396,9 +401,22 @@ static gms_status validate_call(gms_multisplit_ctx* ctx, const uint32_t* kin, co
     return GMS_SUCCESS;
 }
 
+// The wave-level variant where it was chosen for the context (or where the
+// automatic choice takes it), else the block-wide one.
+static bool use_wms(const gms_multisplit_ctx* ctx, uint32_t n, bool kv) {
+    if (ctx->num_buckets < 2 || ctx->num_buckets > kWmsMaxBuckets) return false;
+    if (ctx->variant == GMS_VARIANT_WMS) return true;
+    if (ctx->variant == GMS_VARIANT_BMS) return false;
+    (void)n;
+    (void)kv;
+    return false;
+}
+
 static gms_status run_ctx(gms_multisplit_ctx* ctx, const uint32_t* kin, const uint32_t* vin,
                           uint32_t* kout, uint32_t* vout, uint32_t n, gms_bucket_desc d,
                           uint32_t* d_offsets, hipStream_t s) {
+    if (use_wms(ctx, n, vin != nullptr))
+        return run_wms_multisplit(ctx->s, kin, vin, kout, vout, n, ctx->num_buckets, d, d_offsets, s);
     if (ctx->num_buckets > 256)
         return run_large_multisplit(ctx->large, ctx->s, kin, vin, kout, vout, n, ctx->num_buckets,
                                     d, d_offsets, s);
@@ -499,6 +517,23 @@ gms_status gms_multisplit_wide(gms_multisplit_ctx* ctx, const void* kin, const v
                        hipMemcpyDeviceToHost, s) != hipSuccess)
         return GMS_ERR_HIP;
     return hip_status(hipStreamSynchronize(s));
+}
+
+gms_status gms_multisplit_set_variant(gms_multisplit_ctx* ctx, int variant) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    if (variant != GMS_VARIANT_AUTO && variant != GMS_VARIANT_BMS && variant != GMS_VARIANT_WMS)
+        return GMS_ERR_INVALID_ARG;
+    if (variant == GMS_VARIANT_WMS && (ctx->num_buckets < 2 || ctx->num_buckets > kWmsMaxBuckets))
+        return GMS_ERR_UNSUPPORTED_BUCKETS;
+    ctx->variant = variant;
+    return GMS_SUCCESS;
+}
+
+uint32_t gms_subproblem_size_wms(uint32_t num_buckets, int key_value, uint32_t* size_block) {
+    (void)key_value;  // one subproblem shape for key-only and key-value
+    if (num_buckets < 2 || num_buckets > kWmsMaxBuckets) return 0;
+    if (size_block) *size_block = wms_subproblem_keys() * (kThreads / 64);
+    return wms_subproblem_keys();
 }
 
 gms_status gms_multisplit_set_profiling(gms_multisplit_ctx* ctx, int enable) {

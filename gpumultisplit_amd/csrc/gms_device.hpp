@@ -9,10 +9,13 @@
 #define GMS_READLANE gms_emu::readlane
 #define GMS_READFIRSTLANE(x) (x)
 #define GMS_WAVE_SYNC() gms_emu::wave_sync()
+#define GMS_BALLOT(p) gms_emu::ballot((p))
 #else
 #include <hip/hip_runtime.h>
 #define GMS_UPDATE_DPP __builtin_amdgcn_update_dpp
 #define GMS_READLANE __builtin_amdgcn_readlane
+// 64-bit mask of the lanes of the wave whose predicate holds.
+#define GMS_BALLOT(p) ((uint64_t)__ballot((p)))
 // A value the whole wave holds alike, moved to a scalar register.
 #define GMS_READFIRSTLANE(x) __builtin_amdgcn_readfirstlane(x)
 // Orders the LDS accesses of one wave before and after this point across its
@@ -196,6 +199,16 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
 // Value of lane 63 broadcast to the wave (the total after an inclusive scan).
 __device__ __forceinline__ uint32_t wave_last(uint32_t x) {
     return (uint32_t)GMS_READLANE((int)x, 63);
+}
+
+// Number of lanes below this one whose bit is set in m (v_mbcnt).
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+#ifdef GMS_HOST_EMULATION
+    const uint32_t lane = threadIdx.x & 63u;
+    return (uint32_t)__builtin_popcountll(m & ((1ull << lane) - 1ull));
+#else
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+#endif
 }
 
 // Exclusive scan across a block of NW waves; `total` receives the block sum.

@@ -143,6 +143,14 @@ gms_status run_wide_sort(Scratch& s, uint32_t* ktmp, uint32_t* vtmp, const uint3
                          uint32_t key_words, uint32_t value_words, uint32_t digit_bits,
                          hipStream_t stream);
 
+// Wave-level multisplit (gms_wms.hip): 2..64 buckets, one subproblem of
+// wms_subproblem_keys() keys per wave.
+constexpr uint32_t kWmsMaxBuckets = 64;
+uint32_t wms_subproblem_keys();
+gms_status run_wms_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin,
+                              uint32_t* kout, uint32_t* vout, uint32_t n, uint32_t num_buckets,
+                              gms_bucket_desc d, uint32_t* d_offsets, hipStream_t stream);
+
 gms_status hip_status(hipError_t e);
 bool ranges_overlap(const void* a, size_t an, const void* b, size_t bn);
 

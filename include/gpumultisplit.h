@@ -156,6 +156,25 @@ gms_status gms_multisplit_wide(gms_multisplit_ctx *ctx, const void *d_key_in,
                                gms_bucket_desc64 desc, uint32_t *d_offsets, uint32_t *h_offsets,
                                void *stream);
 
+/* Kernel family of a context's multisplits (up to 64 buckets).  BMS: the
+ * block-wide kernels, one tile per workgroup (the reference's BMS path,
+ * src/kernels/bms/).  WMS: the wave-level kernels, one subproblem per wave
+ * ranked with wave ballots (the reference's warp-wide path,
+ * src/kernels/wms/wms_prescan.cuh:21-149, wms_postscan.cuh:20-505,
+ * wms_postscan_pairs.cuh:29-538, which its driver main_wms.cu selects by
+ * hand).  AUTO (the default) picks per call from the bucket count and n. */
+typedef enum gms_variant {
+    GMS_VARIANT_AUTO = 0,
+    GMS_VARIANT_BMS = 1,
+    GMS_VARIANT_WMS = 2
+} gms_variant;
+gms_status gms_multisplit_set_variant(gms_multisplit_ctx *ctx, int variant);
+
+/* subproblem_size_wms_key_only / _key_value(num_buckets, size_block&)
+ * (src/api/wms_api.h:25-83): keys per wave subproblem of the wave-level
+ * variant, and keys per workgroup in *size_block; 0 outside 2..64 buckets. */
+uint32_t gms_subproblem_size_wms(uint32_t num_buckets, int key_value, uint32_t *size_block);
+
 /* subproblem_size_bms_key_only / _key_value (src/api/bms_api.h:23-89): keys
  * per tile for a bucket count (the tile of inputs below 2^26 keys, also the
  * shortest tile of any input).  Returns 0 for an unsupported count. */

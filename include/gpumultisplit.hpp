@@ -281,6 +281,25 @@ inline uint32_t subproblem_size_bms_key_value(uint32_t num_buckets) {
     return gms_subproblem_size(num_buckets, 1);
 }
 
+// subproblem_size_wms_key_only / _key_value (src/api/wms_api.h:25-83): keys per
+// wave subproblem of the wave-level variant, keys per workgroup in size_block.
+inline uint32_t subproblem_size_wms_key_only(uint32_t num_buckets, uint32_t& size_block) {
+    size_block = 0;
+    return gms_subproblem_size_wms(num_buckets, 0, &size_block);
+}
+inline uint32_t subproblem_size_wms_key_value(uint32_t num_buckets, uint32_t& size_block) {
+    size_block = 0;
+    return gms_subproblem_size_wms(num_buckets, 1, &size_block);
+}
+
+namespace gms {
+// Block-wide (GMS_VARIANT_BMS), wave-level (GMS_VARIANT_WMS, 2..64 buckets) or
+// automatic (GMS_VARIANT_AUTO) kernels for a context's multisplits.
+inline void set_variant(multisplit_context& context, gms_variant v) {
+    check(gms_multisplit_set_variant(context.handle, (int)v), "set_variant");
+}
+}  // namespace gms
+
 // ---------------------------------------------------------------- radix sort
 // src/api/multisplit_sort.cuh:100-116.
 class ms_sort_context {

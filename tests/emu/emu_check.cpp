@@ -328,6 +328,37 @@ int main(int argc, char** argv) {
         }
     }
     printf("wide done\n");
+
+    // wave-level variant: exact-size buffers, ragged subproblems
+    for (uint32_t B : {2u, 5u, 64u}) {
+        for (uint32_t n : {1u, 1023u, 4096u * 2 + 200}) {
+            for (int kv = 0; kv < 2; ++kv) {
+                Buf kin(n), vin(n), kout(n), vout(n);
+                for (uint32_t i = 0; i < n; ++i) {
+                    kin.p[i] = rnd();
+                    vin.p[i] = i;
+                }
+                gms_multisplit_ctx* c;
+                gms_multisplit_create(B, &c);
+                gms_multisplit_allocate(c, n, kv);
+                EXPECT(gms_multisplit_set_variant(c, GMS_VARIANT_WMS) == GMS_SUCCESS, "wms variant");
+                std::vector<uint32_t> offs(B), wk(n), wv(n), wo(B);
+                const gms_bucket_desc d{GMS_BUCKET_RANGE, 0, 0};
+                gms_status st = kv ? gms_multisplit_key_value(c, kin.p, vin.p, kout.p, vout.p, n, d,
+                                                              offs.data(), nullptr)
+                                   : gms_multisplit_key_only(c, kin.p, kout.p, n, d, offs.data(),
+                                                             nullptr);
+                EXPECT(st == GMS_SUCCESS, "wms status B=%u n=%u", B, n);
+                ms_oracle_multisplit(0, B, 0, kin.p, kv ? vin.p : nullptr, wk.data(),
+                                     kv ? wv.data() : nullptr, n, wo.data());
+                EXPECT(memcmp(kout.p, wk.data(), (size_t)n * 4) == 0 && offs == wo &&
+                           (!kv || memcmp(vout.p, wv.data(), (size_t)n * 4) == 0),
+                       "wms B=%u n=%u kv=%d", B, n, kv);
+                gms_multisplit_destroy(c);
+            }
+        }
+    }
+    printf("wms done\n");
     printf(failures ? "EMU CHECK FAILED (%d)\n" : "EMU CHECK OK\n", failures);
     return failures ? 1 : 0;
 }

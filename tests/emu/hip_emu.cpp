@@ -64,6 +64,18 @@ int readlane(int x, int lane) {
     return r;
 }
 
+uint64_t ballot(bool pred) {
+    const uint32_t wave_base = thread_idx.x & ~63u;
+    uint32_t* buf = g_wave_buf + wave_base;
+    buf[thread_idx.x & 63u] = pred ? 1u : 0u;
+    wave_barrier().arrive_and_wait();
+    uint64_t m = 0;
+    const uint32_t lanes = block_dim.x - wave_base < 64u ? block_dim.x - wave_base : 64u;
+    for (uint32_t l = 0; l < lanes; ++l) m |= (uint64_t)(buf[l] & 1u) << l;
+    wave_barrier().arrive_and_wait();
+    return m;
+}
+
 void launch(dim3 grid, dim3 block, std::function<void()> body) {
     std::lock_guard<std::mutex> lk(g_launch_mutex);
     const uint32_t nt = block.x;

@@ -51,6 +51,7 @@ extern gms_emu_dim3 block_idx, block_dim, grid_dim;
 void syncthreads();
 uint32_t update_dpp(uint32_t old, uint32_t src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl);
 int readlane(int x, int lane);
+uint64_t ballot(bool pred);
 void wave_sync();
 void launch(dim3 grid, dim3 block, std::function<void()> body);
 }  // namespace gms_emu

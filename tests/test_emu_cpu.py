@@ -425,3 +425,71 @@ def test_emulated_sort_wide(emu, kbytes, vbytes):
     np.testing.assert_array_equal(kout, keys[order])
     if vbytes:
         np.testing.assert_array_equal(vout, vals[order])
+
+
+# ------------------------------------------------- wave-level variant (WMS)
+def emu_multisplit_variant(L, keys, B, functor, kv, variant):
+    n = keys.size
+    h = ctypes.c_void_p()
+    assert L.gms_multisplit_create(B, ctypes.byref(h)) == 0
+    try:
+        assert L.gms_multisplit_allocate(h, n, int(kv)) == 0
+        assert L.gms_multisplit_set_variant(h, variant) == 0
+        vals = np.arange(n, dtype=np.uint32)
+        kout = np.full(max(n, 1), 0x5A5A5A5A, np.uint32)
+        vout = np.full(max(n, 1), 0x5A5A5A5A, np.uint32)
+        offs = np.zeros(B, np.uint32)
+        if kv:
+            st = L.gms_multisplit_key_value(h, p(keys), p(vals), p(kout), p(vout), n,
+                                            functor.desc(), p(offs), None)
+        else:
+            st = L.gms_multisplit_key_only(h, p(keys), p(kout), n, functor.desc(), p(offs), None)
+        assert st == 0
+        return kout[:n], (vout[:n] if kv else None), offs
+    finally:
+        L.gms_multisplit_destroy(h)
+
+
+@pytest.mark.parametrize("B", [2, 3, 8, 17, 32, 64])
+@pytest.mark.parametrize("kv", [False, True])
+def test_emulated_wms_matches_oracle(emu, B, kv):
+    sub, blk = 1024, 4096
+    for n in (0, 1, 63, 65, sub, sub + 1, blk + 3 * 64 + 5, 2 * blk + 1):
+        keys = W.uniform_keys(n, seed=B * 13 + n)
+        got = emu_multisplit_variant(emu, keys, B, gms.range_bucket(B), kv, gms.VARIANT_WMS)
+        wk, wv, wo = oracle.multisplit(keys, B, oracle.RANGE, 0,
+                                       np.arange(n, dtype=np.uint32) if kv else None)
+        np.testing.assert_array_equal(got[0], wk)
+        np.testing.assert_array_equal(got[2], wo)
+        if kv:
+            np.testing.assert_array_equal(got[1], wv)
+
+
+def test_emulated_wms_other_identifiers_and_skew(emu):
+    n = 3 * 4096 + 77
+    cases = [(64, gms.delta_func(1 << 26), oracle.DELTA, 1 << 26, W.uniform_keys(n, seed=1)),
+             (16, gms.ms_bucket_sort_4bit(8), oracle.DIGIT, 8, W.uniform_keys(n, seed=2)),
+             (40, gms.identity_bucket(), oracle.IDENTITY, 0,
+              W.uniform_keys(n, seed=3) % np.uint32(45)),
+             (32, gms.test_bucket(5), oracle.LOWBITS, 5, W.uniform_keys(n, seed=4)),
+             (64, gms.range_bucket(64), oracle.RANGE, 0, W.keys_for("onehot90", n, 64, seed=5))]
+    for B, f, kind, param, keys in cases:
+        keys = keys.astype(np.uint32)
+        got = emu_multisplit_variant(emu, keys, B, f, True, gms.VARIANT_WMS)
+        wk, wv, wo = oracle.multisplit(keys, B, kind, param, np.arange(n, dtype=np.uint32))
+        np.testing.assert_array_equal(got[0], wk)
+        np.testing.assert_array_equal(got[1], wv)
+        np.testing.assert_array_equal(got[2], wo)
+
+
+def test_wms_variant_validation(emu):
+    h = ctypes.c_void_p()
+    assert emu.gms_multisplit_create(65, ctypes.byref(h)) == 0
+    assert emu.gms_multisplit_set_variant(h, gms.VARIANT_WMS) == 2  # more than 64 buckets
+    assert emu.gms_multisplit_set_variant(h, 7) == 1
+    assert emu.gms_multisplit_set_variant(h, gms.VARIANT_BMS) == 0
+    emu.gms_multisplit_destroy(h)
+    blk = ctypes.c_uint32()
+    assert emu.gms_subproblem_size_wms(64, 1, ctypes.byref(blk)) == 1024 and blk.value == 4096
+    assert emu.gms_subproblem_size_wms(65, 0, ctypes.byref(blk)) == 0
+    assert emu.gms_subproblem_size_wms(1, 0, ctypes.byref(blk)) == 0
