@@ -132,6 +132,8 @@ _SIGS = [
     ("gms_compaction_release", _I, [_P]),
     ("gms_compaction_destroy", _I, [_P]),
     ("gms_compaction", _I, [_P, _P, _P, _P, _P, _U32, gms_bucket_desc, _P, _P, _P]),
+    ("gms_compaction_wide", _I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, gms_bucket_desc64, _P, _P,
+                                 _P]),
     ("gms_histogram_u32", _I, [_P, _U32, _P, _U32, gms_bucket_desc, _P]),
     ("gms_histogram_even_f32", _I, [_P, _U32, _P, _U32, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_float, _P]),
@@ -663,34 +665,45 @@ def compaction_release_memory(context: compaction_context) -> None:
     _check(lib().gms_compaction_release(context.handle), "compaction_release_memory")
 
 
+def _compaction(d_key_in, d_value_in, d_key_out, d_value_out, num_elements, context,
+                bucket_identifier, h_count, d_count, what):
+    st = _stream(d_key_in, d_key_out)
+    if _is_wide(d_key_in, d_value_in, d_key_out, d_value_out):
+        kb = _elem_bytes(d_key_in, d_key_out)
+        vb = 0 if d_value_in is None and d_value_out is None else _elem_bytes(d_value_in, d_value_out)
+        _check(lib().gms_compaction_wide(context.handle, _ptr(d_key_in, (4, 8)),
+                                         _ptr(d_value_in, (4, 8)), _ptr(d_key_out, (4, 8)),
+                                         _ptr(d_value_out, (4, 8)), int(num_elements), kb, vb,
+                                         bucket_identifier.desc64(), h_count, d_count, st), what)
+    else:
+        _check(lib().gms_compaction(context.handle, _ptr(d_key_in), _ptr(d_value_in),
+                                    _ptr(d_key_out), _ptr(d_value_out), int(num_elements),
+                                    bucket_identifier.desc(), h_count, d_count, st), what)
+
+
 def compaction_key_only(d_key_in, d_key_out, num_elements: int, context: compaction_context,
                         bucket_identifier) -> int:
-    """Stable 2-bucket multisplit; returns the bucket-0 count (blocking)."""
+    """Stable 2-bucket multisplit; returns the bucket-0 count (blocking).
+    32- or 64-bit keys."""
     cnt = ctypes.c_uint32()
-    _check(lib().gms_compaction(context.handle, _ptr(d_key_in), None, _ptr(d_key_out), None,
-                                int(num_elements), bucket_identifier.desc(),
-                                ctypes.addressof(cnt), None, _stream(d_key_in, d_key_out)),
-           "compaction_key_only")
+    _compaction(d_key_in, None, d_key_out, None, num_elements, context, bucket_identifier,
+                ctypes.addressof(cnt), None, "compaction_key_only")
     return int(cnt.value)
 
 
 def compaction_key_value(d_key_in, d_value_in, d_key_out, d_value_out, num_elements: int,
                          context: compaction_context, bucket_identifier) -> int:
     cnt = ctypes.c_uint32()
-    _check(lib().gms_compaction(context.handle, _ptr(d_key_in), _ptr(d_value_in), _ptr(d_key_out),
-                                _ptr(d_value_out), int(num_elements), bucket_identifier.desc(),
-                                ctypes.addressof(cnt), None, _stream(d_key_in, d_key_out)),
-           "compaction_key_value")
+    _compaction(d_key_in, d_value_in, d_key_out, d_value_out, num_elements, context,
+                bucket_identifier, ctypes.addressof(cnt), None, "compaction_key_value")
     return int(cnt.value)
 
 
 def compaction_device_count(d_key_in, d_key_out, num_elements: int, context: compaction_context,
                             bucket_identifier, d_count, d_value_in=None, d_value_out=None) -> None:
     """Non-blocking compaction: the bucket-0 count goes to device memory."""
-    _check(lib().gms_compaction(context.handle, _ptr(d_key_in), _ptr(d_value_in), _ptr(d_key_out),
-                                _ptr(d_value_out), int(num_elements), bucket_identifier.desc(),
-                                None, _ptr(d_count), _stream(d_key_in, d_key_out)),
-           "compaction")
+    _compaction(d_key_in, d_value_in, d_key_out, d_value_out, num_elements, context,
+                bucket_identifier, None, _ptr(d_count), "compaction")
 
 
 # ---------------------------------------------------------------------------

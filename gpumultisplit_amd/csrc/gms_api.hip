@@ -15,6 +15,10 @@ struct gms_multisplit_ctx {
     uint32_t capacity = 0;
     int allocated = 0;
     int variant = GMS_VARIANT_AUTO;  // block-wide or wave-level kernels
+    // 64-bit elements into more than 256 buckets: pass temporaries of
+    // capacity x 2 words each, allocated by the first such call
+    uint32_t* wide_k = nullptr;
+    uint32_t* wide_v = nullptr;
     Scratch s;
     LargeScratch large;  // more than 256 buckets only
     Profiler prof;
@@ -366,6 +370,9 @@ gms_status gms_multisplit_allocate(gms_multisplit_ctx* ctx, uint32_t n_capacity,
 
 gms_status gms_multisplit_release(gms_multisplit_ctx* ctx) {
     if (!ctx) return GMS_ERR_INVALID_ARG;
+    if (ctx->wide_k) (void)hipFree(ctx->wide_k);
+    if (ctx->wide_v) (void)hipFree(ctx->wide_v);
+    ctx->wide_k = ctx->wide_v = nullptr;
     scratch_free(ctx->s);
     large_scratch_free(ctx->large);
     ctx->prof.destroy();
@@ -494,7 +501,6 @@ gms_status gms_multisplit_wide(gms_multisplit_ctx* ctx, const void* kin, const v
     }
     if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
     if (n > ctx->capacity) return GMS_ERR_CAPACITY;
-    if (ctx->num_buckets > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;
     st = check_desc64(ctx->num_buckets, d, 32 * kw);
     if (st != GMS_SUCCESS) return st;
     if (n) {
@@ -508,10 +514,30 @@ gms_status gms_multisplit_wide(gms_multisplit_ctx* ctx, const void* kin, const v
                    ranges_overlap(vin, vb, kout, kb) || ranges_overlap(kout, kb, vout, vb)))
             return GMS_ERR_ALIAS;
     }
-    uint32_t* offs = d_offsets ? d_offsets : (h_offsets ? ctx->s.offs : nullptr);
-    st = run_wide_multisplit(ctx->s, static_cast<const uint32_t*>(kin),
-                             static_cast<const uint32_t*>(vin), static_cast<uint32_t*>(kout),
-                             static_cast<uint32_t*>(vout), n, kw, vw, ctx->num_buckets, d, offs, s);
+    const bool large = ctx->num_buckets > 256;
+    uint32_t* staging = large ? ctx->large.offs : ctx->s.offs;
+    uint32_t* offs = d_offsets ? d_offsets : (h_offsets ? staging : nullptr);
+    if (large) {
+        if (!ctx->wide_k) {
+            const size_t bytes = std::max<size_t>(ctx->capacity, 1) * 2 * sizeof(uint32_t);
+            if (hipMalloc((void**)&ctx->wide_k, bytes) != hipSuccess) ctx->wide_k = nullptr;
+            if (hipMalloc((void**)&ctx->wide_v, bytes) != hipSuccess) ctx->wide_v = nullptr;
+            if (!ctx->wide_k || !ctx->wide_v) {
+                if (ctx->wide_k) (void)hipFree(ctx->wide_k);
+                if (ctx->wide_v) (void)hipFree(ctx->wide_v);
+                ctx->wide_k = ctx->wide_v = nullptr;
+                return GMS_ERR_ALLOC;
+            }
+        }
+        st = run_wide_large(ctx->s, ctx->wide_k, ctx->wide_v, static_cast<const uint32_t*>(kin),
+                            static_cast<const uint32_t*>(vin), static_cast<uint32_t*>(kout),
+                            static_cast<uint32_t*>(vout), n, kw, vw, ctx->num_buckets, d, offs, s);
+    } else {
+        st = run_wide_multisplit(ctx->s, static_cast<const uint32_t*>(kin),
+                                 static_cast<const uint32_t*>(vin), static_cast<uint32_t*>(kout),
+                                 static_cast<uint32_t*>(vout), n, kw, vw, ctx->num_buckets, d,
+                                 offs, s);
+    }
     if (st != GMS_SUCCESS || !h_offsets) return st;
     if (hipMemcpyAsync(h_offsets, offs, (size_t)ctx->num_buckets * sizeof(uint32_t),
                        hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -797,6 +823,54 @@ gms_status gms_compaction(gms_compaction_ctx* ctx, const uint32_t* kin, const ui
     hipStream_t s = (hipStream_t)stream;
     // Offsets {0, count0} land in the scratch's offset slots; with n == 0 they are zero.
     st = run_multisplit(ctx->s, kin, vin, kout, vout, n, 2, d, ctx->s.offs, s, nullptr);
+    if (st != GMS_SUCCESS) return st;
+    if (d_count &&
+        hipMemcpyAsync(d_count, ctx->s.offs + 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s) !=
+            hipSuccess)
+        return GMS_ERR_HIP;
+    if (h_count) {
+        if (hipMemcpyAsync(h_count, ctx->s.offs + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s) !=
+            hipSuccess)
+            return GMS_ERR_HIP;
+        return hip_status(hipStreamSynchronize(s));
+    }
+    return GMS_SUCCESS;
+}
+
+gms_status gms_compaction_wide(gms_compaction_ctx* ctx, const void* kin, const void* vin,
+                               void* kout, void* vout, uint32_t n, uint32_t key_bytes,
+                               uint32_t value_bytes, gms_bucket_desc64 d, uint32_t* h_count,
+                               uint32_t* d_count, void* stream) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    if ((key_bytes != 4 && key_bytes != 8) || (value_bytes != 0 && value_bytes != 4 && value_bytes != 8))
+        return GMS_ERR_INVALID_ARG;
+    if (!value_bytes && (vin || vout)) return GMS_ERR_INVALID_ARG;
+    if (value_bytes && n && (!vin || !vout)) return GMS_ERR_INVALID_ARG;
+    const uint32_t kw = key_bytes / 4, vw = value_bytes / 4;
+    gms_status st = check_desc64(2, d, 32 * kw);
+    if (st != GMS_SUCCESS) return st;
+    if (kw == 1 && vw <= 1)
+        return gms_compaction(ctx, static_cast<const uint32_t*>(kin), static_cast<const uint32_t*>(vin),
+                              static_cast<uint32_t*>(kout), static_cast<uint32_t*>(vout), n,
+                              gms_bucket_desc{d.kind, (uint32_t)d.param, d.width}, h_count, d_count,
+                              stream);
+    if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
+    if (n > ctx->capacity) return GMS_ERR_CAPACITY;
+    if (n) {
+        if (!kin || !kout) return GMS_ERR_INVALID_ARG;
+        if ((kw == 2 && (!aligned8(kin) || !aligned8(kout))) ||
+            (vw == 2 && (!aligned8(vin) || !aligned8(vout))))
+            return GMS_ERR_INVALID_ARG;
+        const size_t kb = (size_t)n * key_bytes, vb = (size_t)n * value_bytes;
+        if (ranges_overlap(kin, kb, kout, kb) ||
+            (vw && (ranges_overlap(vin, vb, vout, vb) || ranges_overlap(kin, kb, vout, vb) ||
+                    ranges_overlap(vin, vb, kout, kb) || ranges_overlap(kout, kb, vout, vb))))
+            return GMS_ERR_ALIAS;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    st = run_wide_multisplit(ctx->s, static_cast<const uint32_t*>(kin),
+                             static_cast<const uint32_t*>(vin), static_cast<uint32_t*>(kout),
+                             static_cast<uint32_t*>(vout), n, kw, vw, 2, d, ctx->s.offs, s);
     if (st != GMS_SUCCESS) return st;
     if (d_count &&
         hipMemcpyAsync(d_count, ctx->s.offs + 1, sizeof(uint32_t), hipMemcpyDeviceToDevice, s) !=

@@ -173,6 +173,13 @@ struct AnyF64 {
     }
 };
 
+// One digit of a 64-bit key's bucket id (the 64-bit form of DigitOfF below).
+struct DigitOf64F {
+    AnyF64 base;
+    uint32_t shift, mask;
+    GMS_HD uint32_t operator()(uint64_t k) const { return (base(k) >> shift) & mask; }
+};
+
 // One digit of another identifier's bucket id: (base(k) >> shift) & mask,
 // with base's clamp applied first.  The LSD passes of a multisplit into more
 // than 256 buckets rank by these digits of the (up to 20-bit) bucket id.

@@ -136,6 +136,13 @@ gms_status run_wide_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* 
                                uint32_t value_words, uint32_t num_buckets,
                                const gms_bucket_desc64& d, uint32_t* d_offsets,
                                hipStream_t stream);
+// The same into more than 256 buckets (up to kMaxBuckets): LSD passes over
+// 8-bit digits of the bucket id, ping-ponging through ktmp/vtmp (n x key_words
+// and n x value_words words).
+gms_status run_wide_large(Scratch& s, uint32_t* ktmp, uint32_t* vtmp, const uint32_t* kin,
+                          const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint32_t n,
+                          uint32_t key_words, uint32_t value_words, uint32_t num_buckets,
+                          const gms_bucket_desc64& d, uint32_t* d_offsets, hipStream_t stream);
 // Passes of the radix sort of key_words-word keys (p holds up to 16).
 int wide_sort_passes(uint32_t key_words, uint32_t digit_bits, SortPass* p);
 gms_status run_wide_sort(Scratch& s, uint32_t* ktmp, uint32_t* vtmp, const uint32_t* kin,

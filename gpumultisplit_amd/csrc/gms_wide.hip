@@ -250,10 +250,13 @@ static hipError_t wide_stage(const WideCall& c, int log_b, const F& f, bool pres
     }
 }
 
+// Stage launch for the element shape; mk64 / mk32 build the identifier of a
+// 64-bit / 32-bit key (only the one the shape uses is built).
+template <class Mk64, class Mk32>
 static hipError_t wide_dispatch(const WideCall& c, int log_b, uint32_t kw, uint32_t vw,
-                                uint32_t B, const gms_bucket_desc64& d, bool prescan) {
+                                const Mk64& mk64, const Mk32& mk32, bool prescan) {
     if (kw == 2) {
-        const AnyF64 f = AnyF64::make(d.kind, d.param, d.width, B);
+        const auto f = mk64();
         switch (vw) {
             case 0: return wide_stage<2, 0>(c, log_b, f, prescan);
             case 1: return wide_stage<2, 1>(c, log_b, f, prescan);
@@ -262,18 +265,18 @@ static hipError_t wide_dispatch(const WideCall& c, int log_b, uint32_t kw, uint3
         }
     }
     if (vw != 2) return hipErrorInvalidValue;
-    const AnyF f = AnyF::make(d.kind, (uint32_t)d.param, d.width, B);
-    return wide_stage<1, 2>(c, log_b, f, prescan);
+    return wide_stage<1, 2>(c, log_b, mk32(), prescan);
 }
 
 uint32_t wide_tile_elems(uint32_t key_words, uint32_t value_words) {
     return kThreads * (uint32_t)wide_items((int)key_words, (int)value_words);
 }
 
-gms_status run_wide_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin,
-                               uint32_t* kout, uint32_t* vout, uint32_t n, uint32_t kw,
-                               uint32_t vw, uint32_t B, const gms_bucket_desc64& d,
-                               uint32_t* d_offsets, hipStream_t stream) {
+template <class Mk64, class Mk32>
+static gms_status wide_stages(Scratch& s, const uint32_t* kin, const uint32_t* vin,
+                              uint32_t* kout, uint32_t* vout, uint32_t n, uint32_t kw,
+                              uint32_t vw, uint32_t B, const Mk64& mk64, const Mk32& mk32,
+                              uint32_t* d_offsets, hipStream_t stream) {
     if (n == 0) return d_offsets ? hip_status(launch_fill(d_offsets, 0u, B, stream)) : GMS_SUCCESS;
     if (B == 1) {
         hipError_t e = launch_copy(kin, kout, (size_t)n * kw, stream);
@@ -288,11 +291,76 @@ gms_status run_wide_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* 
     if (m > s.hist_cap || scan_partials_needed(m) > s.part_cap) return GMS_ERR_CAPACITY;
     auto al16 = [](const void* p) { return p == nullptr || (((uintptr_t)p) & 15u) == 0; };
     const WideCall c{kin, vin, kout, vout, n, nt, s.hist, al16(kin) && al16(vin), stream};
-    hipError_t e = wide_dispatch(c, lb, kw, vw, B, d, true);
+    hipError_t e = wide_dispatch(c, lb, kw, vw, mk64, mk32, true);
     if (e == hipSuccess) e = launch_exclusive_scan(s.hist, s.hist, (uint32_t)m, s.part, stream);
-    if (e == hipSuccess) e = wide_dispatch(c, lb, kw, vw, B, d, false);
+    if (e == hipSuccess) e = wide_dispatch(c, lb, kw, vw, mk64, mk32, false);
     if (e == hipSuccess && d_offsets) e = launch_gather_offsets(s.hist, nt, B, d_offsets, stream);
     return hip_status(e);
+}
+
+gms_status run_wide_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin,
+                               uint32_t* kout, uint32_t* vout, uint32_t n, uint32_t kw,
+                               uint32_t vw, uint32_t B, const gms_bucket_desc64& d,
+                               uint32_t* d_offsets, hipStream_t stream) {
+    return wide_stages(
+        s, kin, vin, kout, vout, n, kw, vw, B,
+        [&] { return AnyF64::make(d.kind, d.param, d.width, B); },
+        [&] { return AnyF::make(d.kind, (uint32_t)d.param, d.width, B); }, d_offsets, stream);
+}
+
+// Bucket starts of a multisplit output (more than 256 buckets): thread i
+// writes the starts of the buckets in (f(key[i-1]), f(key[i])], the last
+// element's thread also those above f(key[n-1]).
+template <int KW, class F>
+__global__ __launch_bounds__(kThreads) void wide_bucket_starts_kernel(const uint32_t* __restrict__ kout,
+                                                                      uint32_t n, uint32_t num_buckets,
+                                                                      F f, uint32_t* __restrict__ start) {
+    const size_t stride = (size_t)gridDim.x * kThreads;
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
+        const uint32_t hi = f(key_word_of<KW>(kout + i * KW, 0));
+        const uint32_t lo = i == 0 ? 0u : f(key_word_of<KW>(kout + (i - 1) * KW, 0)) + 1u;
+        for (uint32_t b = lo; b <= hi && b < num_buckets; ++b) start[b] = (uint32_t)i;
+        if (i + 1 == n)
+            for (uint32_t b = hi + 1u; b < num_buckets; ++b) start[b] = n;
+    }
+}
+
+gms_status run_wide_large(Scratch& s, uint32_t* ktmp, uint32_t* vtmp, const uint32_t* kin,
+                          const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint32_t n,
+                          uint32_t kw, uint32_t vw, uint32_t B, const gms_bucket_desc64& d,
+                          uint32_t* d_offsets, hipStream_t stream) {
+    if (n == 0) return d_offsets ? hip_status(launch_fill(d_offsets, 0u, B, stream)) : GMS_SUCCESS;
+    const int bits = ceil_log2_u32(B);
+    const int passes = (bits + 7) / 8;
+    const uint32_t* sk = kin;
+    const uint32_t* sv = vin;
+    for (int p = 0; p < passes; ++p) {
+        const uint32_t shift = 8u * (uint32_t)p;
+        const uint32_t w = (uint32_t)(bits - (int)shift < 8 ? bits - (int)shift : 8);
+        const bool to_out = ((passes - 1 - p) % 2) == 0;  // the last pass writes the output
+        uint32_t* dk = to_out ? kout : ktmp;
+        uint32_t* dv = vw ? (to_out ? vout : vtmp) : nullptr;
+        const uint32_t mask = (1u << w) - 1u;
+        gms_status st = wide_stages(
+            s, sk, sv, dk, dv, n, kw, vw, 1u << w,
+            [&] { return DigitOf64F{AnyF64::make(d.kind, d.param, d.width, B), shift, mask}; },
+            [&] { return DigitOfF{AnyF::make(d.kind, (uint32_t)d.param, d.width, B), shift, mask}; },
+            nullptr, stream);
+        if (st != GMS_SUCCESS) return st;
+        sk = dk;
+        sv = dv;
+    }
+    if (!d_offsets) return GMS_SUCCESS;
+    const uint32_t blocks = (uint32_t)std::min<size_t>(((size_t)n + kThreads - 1) / kThreads, 8192);
+    if (kw == 2)
+        hipLaunchKernelGGL((wide_bucket_starts_kernel<2, AnyF64>), dim3(blocks), dim3(kThreads), 0,
+                           stream, (const uint32_t*)kout, n, B,
+                           AnyF64::make(d.kind, d.param, d.width, B), d_offsets);
+    else
+        hipLaunchKernelGGL((wide_bucket_starts_kernel<1, AnyF>), dim3(blocks), dim3(kThreads), 0,
+                           stream, (const uint32_t*)kout, n, B,
+                           AnyF::make(d.kind, (uint32_t)d.param, d.width, B), d_offsets);
+    return hip_status(hipGetLastError());
 }
 
 int wide_sort_passes(uint32_t key_words, uint32_t digit_bits, SortPass* p) {

@@ -147,7 +147,9 @@ gms_status gms_multisplit(gms_multisplit_ctx *ctx, const uint32_t *d_key_in,
  * key_bytes 4 or 8; value_bytes 0 (key-only), 4 or 8.  Arrays of 8-byte
  * elements must be 8-byte aligned.  With 4-byte keys and values (or none) this
  * is gms_multisplit / gms_multisplit_key_value; any 8-byte operand takes the
- * 64-bit kernels, which support up to 256 buckets.  Bucket starts go to
+ * 64-bit kernels: one pass up to 256 buckets, LSD passes over 8-bit digits of
+ * the bucket id above (up to 2^20 buckets; the first such call on a context
+ * allocates its pass temporaries, 2 x capacity x 8 bytes).  Bucket starts go to
  * d_offsets (device, asynchronous) and/or h_offsets (host; the call then
  * blocks), num_buckets entries each.  The context's allocation serves both. */
 gms_status gms_multisplit_wide(gms_multisplit_ctx *ctx, const void *d_key_in,
@@ -276,6 +278,16 @@ gms_status gms_compaction(gms_compaction_ctx *ctx, const uint32_t *d_key_in,
                           const uint32_t *d_value_in, uint32_t *d_key_out,
                           uint32_t *d_value_out, uint32_t n, gms_bucket_desc desc,
                           uint32_t *h_count, uint32_t *d_count, void *stream);
+
+/* The same for 32- or 64-bit keys and 32- or 64-bit values (the reference's
+ * compaction_key_*<key_t, value_t> templates for 8-byte types,
+ * multisplit2_compaction.cuh:525-573): key_bytes 4 or 8, value_bytes 0, 4 or
+ * 8; 8-byte arrays 8-byte aligned; the identifier as a gms_bucket_desc64. */
+gms_status gms_compaction_wide(gms_compaction_ctx *ctx, const void *d_key_in,
+                               const void *d_value_in, void *d_key_out, void *d_value_out,
+                               uint32_t n, uint32_t key_bytes, uint32_t value_bytes,
+                               gms_bucket_desc64 desc, uint32_t *h_count, uint32_t *d_count,
+                               void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Histogram                                                                 */

@@ -20,10 +20,10 @@
 //    specialising gms::bucket_traits<T> with
 //      static gms_bucket_desc desc(const T&, uint32_t num_buckets);
 //    mapping it to one of the built-in kinds.
-//  * KeyT and ValueT of multisplits and sorts may be 32- or 64-bit (any 8-byte
-//    operand takes the 64-bit kernels, up to 256 buckets); compaction and
-//    histograms take 32-bit keys.  A 64-bit identifier parameter (a delta
-//    above 2^32) needs gms::delta_func64.  Sorts of 64-bit keys or values are
+//  * KeyT and ValueT of multisplits, sorts and compactions may be 32- or
+//    64-bit (any 8-byte operand takes the 64-bit kernels); histograms take
+//    32-bit keys.  A 64-bit identifier parameter (a delta above 2^32) needs
+//    gms::delta_func64.  Sorts of 64-bit keys or values are
 //    allocated through the typed forms, e.g.
 //    multisplit_sort_8bit_allocate_pairs<uint64_t, uint64_t>(n, ctx).
 #ifndef GPUMULTISPLIT_HPP_
@@ -400,12 +400,18 @@ template <typename key_t, typename bucket_t>
 inline unsigned compaction_key_only(key_t* d_key_in, key_t* d_key_out, uint32_t num_elements,
                                     compaction_context& context, bucket_t bucket_identifier,
                                     void* stream = nullptr) {
-    gms::require_32bit<key_t>("key");
-    uint32_t count = 0;
-    gms::check(gms_compaction(context.handle, reinterpret_cast<const uint32_t*>(d_key_in), nullptr,
-                              reinterpret_cast<uint32_t*>(d_key_out), nullptr, num_elements,
-                              gms::desc_of(bucket_identifier, 2), &count, nullptr, stream),
-               "compaction_key_only");
+    unsigned count = 0;
+    if constexpr (gms::elem_bytes<key_t>() == 8) {
+        gms::check(gms_compaction_wide(context.handle, d_key_in, nullptr, d_key_out, nullptr,
+                                       num_elements, 8, 0, gms::desc64_of(bucket_identifier, 2),
+                                       &count, nullptr, stream),
+                   "compaction_key_only");
+    } else {
+        gms::check(gms_compaction(context.handle, reinterpret_cast<const uint32_t*>(d_key_in), nullptr,
+                                  reinterpret_cast<uint32_t*>(d_key_out), nullptr, num_elements,
+                                  gms::desc_of(bucket_identifier, 2), &count, nullptr, stream),
+                   "compaction_key_only");
+    }
     return count;
 }
 
@@ -415,15 +421,21 @@ inline unsigned compaction_key_value(key_t* d_key_in, value_t* d_value_in, key_t
                                      value_t* d_value_out, uint32_t num_elements,
                                      compaction_context& context, bucket_t bucket_identifier,
                                      void* stream = nullptr) {
-    gms::require_32bit<key_t>("key");
-    gms::require_32bit<value_t>("value");
-    uint32_t count = 0;
-    gms::check(gms_compaction(context.handle, reinterpret_cast<const uint32_t*>(d_key_in),
-                              reinterpret_cast<const uint32_t*>(d_value_in),
-                              reinterpret_cast<uint32_t*>(d_key_out),
-                              reinterpret_cast<uint32_t*>(d_value_out), num_elements,
-                              gms::desc_of(bucket_identifier, 2), &count, nullptr, stream),
-               "compaction_key_value");
+    unsigned count = 0;
+    constexpr uint32_t kb = gms::elem_bytes<key_t>(), vb = gms::elem_bytes<value_t>();
+    if constexpr (kb == 8 || vb == 8) {
+        gms::check(gms_compaction_wide(context.handle, d_key_in, d_value_in, d_key_out, d_value_out,
+                                       num_elements, kb, vb, gms::desc64_of(bucket_identifier, 2),
+                                       &count, nullptr, stream),
+                   "compaction_key_value");
+    } else {
+        gms::check(gms_compaction(context.handle, reinterpret_cast<const uint32_t*>(d_key_in),
+                                  reinterpret_cast<const uint32_t*>(d_value_in),
+                                  reinterpret_cast<uint32_t*>(d_key_out),
+                                  reinterpret_cast<uint32_t*>(d_value_out), num_elements,
+                                  gms::desc_of(bucket_identifier, 2), &count, nullptr, stream),
+                   "compaction_key_value");
+    }
     return count;
 }
 

@@ -493,3 +493,47 @@ def test_wms_variant_validation(emu):
     assert emu.gms_subproblem_size_wms(64, 1, ctypes.byref(blk)) == 1024 and blk.value == 4096
     assert emu.gms_subproblem_size_wms(65, 0, ctypes.byref(blk)) == 0
     assert emu.gms_subproblem_size_wms(1, 0, ctypes.byref(blk)) == 0
+
+
+@pytest.mark.parametrize("kbytes,vbytes", [(8, 0), (8, 8), (4, 8)])
+def test_emulated_wide_more_than_256_buckets(emu, kbytes, vbytes):
+    for B, kind, param, width in [(1000, gms.BUCKET_RANGE, 0, 0), (4096, gms.BUCKET_DIGIT, 20, 12),
+                                  (70000, gms.BUCKET_DELTA, ((1 << 64) - 1) // 65000 if kbytes == 8 else 60000, 0)]:
+        if kbytes == 4 and kind == gms.BUCKET_DIGIT:
+            param = 10
+        for n in (0, 1, 2 * 3072 + 17):
+            keys = _wide_keys(n, seed=n + B, kbytes=kbytes)
+            vals = None
+            if vbytes:
+                vals = (np.arange(n, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)).astype(
+                    np.uint64 if vbytes == 8 else np.uint32)
+            got = emu_multisplit_wide(emu, keys, vals, B, kind, param, width)
+            wk, wv, wo = oracle.multisplit_wide(keys, B, kind, param, vals, width)
+            np.testing.assert_array_equal(got[0], wk)
+            np.testing.assert_array_equal(got[2], wo)
+            if vbytes:
+                np.testing.assert_array_equal(got[1], wv)
+
+
+@pytest.mark.parametrize("kbytes,vbytes", [(8, 0), (8, 8), (4, 8)])
+def test_emulated_compaction_wide(emu, kbytes, vbytes):
+    n = 2 * 4096 + 33
+    keys = _wide_keys(n, seed=9, kbytes=kbytes)
+    vals = None if not vbytes else np.arange(n, dtype=np.uint64 if vbytes == 8 else np.uint32)
+    h = ctypes.c_void_p()
+    assert emu.gms_compaction_create(ctypes.byref(h)) == 0
+    try:
+        assert emu.gms_compaction_allocate(h, n, 1) == 0
+        kout = np.zeros(n, keys.dtype)
+        vout = None if vals is None else np.zeros(n, vals.dtype)
+        cnt = ctypes.c_uint32()
+        d = gms.gms_bucket_desc64(gms.BUCKET_LOWBITS, 0, 1)
+        assert emu.gms_compaction_wide(h, p(keys), p(vals), p(kout), p(vout), n, kbytes, vbytes, d,
+                                       ctypes.addressof(cnt), None, None) == 0
+    finally:
+        emu.gms_compaction_destroy(h)
+    wk, wv, wo = oracle.multisplit_wide(keys, 2, oracle.LOWBITS, 1, vals)
+    np.testing.assert_array_equal(kout, wk)
+    assert cnt.value == wo[1] == int(np.sum((keys & 1) == 0))
+    if vbytes:
+        np.testing.assert_array_equal(vout, wv)

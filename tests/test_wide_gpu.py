@@ -1,6 +1,7 @@
-"""GPU parity of multisplit and radix sort with 64-bit keys and/or 64-bit
-values (the reference's KeyT/ValueT templates, src/api/multisplit.cuh:81-313,
-multisplit_sort.cuh:366-527, multisplit_sort_pairs.cuh:67-228), bit-exact
+"""GPU parity of multisplit (up to 2^20 buckets), radix sort and compaction
+with 64-bit keys and/or 64-bit values (the reference's KeyT/ValueT templates,
+src/api/multisplit.cuh:81-313, multisplit_sort.cuh:366-527,
+multisplit_sort_pairs.cuh:67-228, multisplit2_compaction.cuh:525-573), bit-exact
 against the oracle's 64-bit restatement (itself pinned against Python
 integers and numpy's stable argsort) and numpy's stable sort."""
 import numpy as np
@@ -163,11 +164,7 @@ def test_sort_wide_matches_numpy(dev, bits, kbytes, vbytes):
 
 def test_wide_rejects_what_it_cannot_do(dev):
     import torch
-    ctx = gms.multisplit_context(1000)  # more than 256 buckets: 32-bit keys only
-    gms.multisplit_allocate_key_only(16, ctx)
     k = torch.zeros(16, dtype=torch.int64, device=dev)
-    with pytest.raises(gms.GmsError):
-        gms.multisplit_key_only(k, torch.zeros_like(k), 16, ctx, gms.range_bucket(1000))
     ctx2 = gms.multisplit_context(256)
     gms.multisplit_allocate_key_only(16, ctx2)
     with pytest.raises(gms.GmsError):  # overlapping input and output
@@ -197,3 +194,48 @@ def test_multisplit_wide_kv_256_2e26(dev):
     # stability: within every bucket the original indices increase
     same = np.diff(ids) == 0
     assert np.all(np.diff(got[1].astype(np.int64))[same] > 0)
+
+
+@pytest.mark.parametrize("kbytes,vbytes", [(8, 0), (8, 4), (8, 8), (4, 8)])
+def test_multisplit_wide_more_than_256_buckets(dev, kbytes, vbytes):
+    for B, kind, param, width in [(257, gms.BUCKET_RANGE, 0, 0), (5000, gms.BUCKET_DIGIT, 30, 12),
+                                  (1 << 20, gms.BUCKET_RANGE, 0, 0),
+                                  (70000, gms.BUCKET_DELTA,
+                                   ((1 << 64) - 1) // 65000 if kbytes == 8 else 60000, 0)]:
+        if kbytes == 4 and kind == gms.BUCKET_DIGIT:
+            param = 8
+        for n in (0, 1, 300_007):
+            banner(f"wide large k{kbytes} v{vbytes} B={B} n={n}")
+            keys = keys_of(n, seed=B + n, kbytes=kbytes)
+            vals = vals_of(n, vbytes)
+            got = run_wide(dev, keys, vals, B, _Ident64(kind, param, width))
+            wk, wv, wo = oracle.multisplit_wide(keys, B, kind, param, vals, width)
+            np.testing.assert_array_equal(got[0], wk)
+            np.testing.assert_array_equal(got[2], wo)
+            if vbytes:
+                np.testing.assert_array_equal(got[1], wv)
+
+
+@pytest.mark.parametrize("kbytes,vbytes", [(8, 0), (8, 8), (4, 8)])
+def test_compaction_wide(dev, kbytes, vbytes):
+    import torch
+    n = 1_000_003
+    keys, vals = keys_of(n, 21, kbytes), vals_of(n, vbytes)
+    ctx = gms.compaction_context()
+    gms.compaction_allocate_key_value(n, ctx)
+    kin, kout = to_dev_any(keys, dev), out_like(keys, dev)
+    f = gms.test_bucket(1)
+    if vals is None:
+        cnt = gms.compaction_key_only(kin, kout, n, ctx, f)
+    else:
+        vout = out_like(vals, dev)
+        cnt = gms.compaction_key_value(kin, to_dev_any(vals, dev), kout, vout, n, ctx, f)
+    wk, wv, wo = oracle.multisplit_wide(keys, 2, oracle.LOWBITS, 1, vals)
+    assert cnt == wo[1] == int(np.sum((keys & 1) == 0))
+    np.testing.assert_array_equal(from_dev_any(kout, keys.dtype)[:n], wk)
+    if vals is not None:
+        np.testing.assert_array_equal(from_dev_any(vout, vals.dtype)[:n], wv)
+    d_cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    gms.compaction_device_count(kin, kout, n, ctx, f, d_cnt)
+    torch.cuda.synchronize()
+    assert int(d_cnt.item()) == cnt
