@@ -137,6 +137,7 @@ _SIGS = [
     ("gms_histogram_u32", _I, [_P, _U32, _P, _U32, gms_bucket_desc, _P]),
     ("gms_histogram_even_f32", _I, [_P, _U32, _P, _U32, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_float, _P]),
+    ("gms_histogram_wide", _I, [_P, _U32, _U32, _P, _U32, gms_bucket_desc64, _P]),
     ("gms_histogram_range_f32", _I, [_P, _U32, _P, _U32, _P, _P]),
     ("gms_histogram_range_u32", _I, [_P, _U32, _P, _U32, _P, _P]),
     ("gms_batch_create", _I, [_PP]),
@@ -710,7 +711,13 @@ def compaction_device_count(d_key_in, d_key_out, num_elements: int, context: com
 # Histogram (src/kernels/histogram/multisplit2_histograms.cuh)
 # ---------------------------------------------------------------------------
 def histogram(d_key_in, num_elements: int, d_histogram, num_bins: int, bucket_identifier) -> None:
-    """Adds bucket-id counts of num_elements keys into d_histogram[num_bins]."""
+    """Adds bucket-id counts of num_elements keys (32- or 64-bit) into
+    d_histogram[num_bins]."""
+    if _is_wide(d_key_in):
+        _check(lib().gms_histogram_wide(_ptr(d_key_in, (8,)), int(num_elements), 8,
+                                        _ptr(d_histogram), int(num_bins),
+                                        bucket_identifier.desc64(), _stream(d_key_in)), "histogram")
+        return
     _check(lib().gms_histogram_u32(_ptr(d_key_in), int(num_elements), _ptr(d_histogram),
                                    int(num_bins), bucket_identifier.desc(), _stream(d_key_in)),
            "histogram")

@@ -908,6 +908,23 @@ gms_status gms_histogram_even_f32(const float* x, uint32_t n, int32_t* d_hist, u
         launch_histogram_even_f32(x, n, d_hist, num_bins, lower, inv_delta, (hipStream_t)stream));
 }
 
+gms_status gms_histogram_wide(const void* d_keys, uint32_t n, uint32_t key_bytes, uint32_t* d_hist,
+                              uint32_t num_bins, gms_bucket_desc64 d, void* stream) {
+    if (key_bytes != 4 && key_bytes != 8) return GMS_ERR_INVALID_ARG;
+    if (num_bins < 1 || num_bins > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    gms_status st = check_desc64(num_bins, d, 8 * key_bytes);
+    if (st != GMS_SUCCESS) return st;
+    if (key_bytes == 4)
+        return gms_histogram_u32(static_cast<const uint32_t*>(d_keys), n, d_hist, num_bins,
+                                 gms_bucket_desc{d.kind, (uint32_t)d.param, d.width}, stream);
+    if (!d_hist) return GMS_ERR_INVALID_ARG;
+    if (n == 0) return GMS_SUCCESS;
+    if (!d_keys || !aligned8(d_keys)) return GMS_ERR_INVALID_ARG;
+    return hip_status(launch_histogram_u64(static_cast<const uint64_t*>(d_keys), n, d_hist,
+                                           num_bins, AnyF64::make(d.kind, d.param, d.width, num_bins),
+                                           (hipStream_t)stream));
+}
+
 gms_status gms_histogram_range_f32(const float* d_samples, uint32_t n, int32_t* d_hist,
                                    uint32_t num_levels, const float* d_levels, void* stream) {
     if (num_levels < 2 || num_levels > 257) return GMS_ERR_UNSUPPORTED_BUCKETS;

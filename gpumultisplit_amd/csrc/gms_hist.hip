@@ -218,6 +218,52 @@ __global__ __launch_bounds__(kThreads) void hist_range_kernel(const uint32_t* __
     });
 }
 
+// 64-bit keys through a 64-bit identifier: each thread counts 4 keys per
+// step with 8-byte loads (u64 arrays are 8-byte aligned) into its wave's LDS
+// sub-histogram; a resident grid walks the array.
+__global__ __launch_bounds__(kThreads) void hist_u64_kernel(const uint2* __restrict__ keys,
+                                                            uint32_t n, uint32_t nbins,
+                                                            uint32_t* __restrict__ d_hist, AnyF64 f) {
+    __shared__ uint32_t s_h[kWaves * 256];
+    const uint32_t tid = threadIdx.x, wid = tid >> 6;
+    for (uint32_t i = tid; i < kWaves * 256u; i += kThreads) s_h[i] = 0;
+    __syncthreads();
+    uint32_t* my = s_h + wid * 256u;
+    const size_t stride = (size_t)gridDim.x * kThreads;
+    size_t e = (size_t)blockIdx.x * kThreads + tid;
+    for (; e + 3 * stride < n; e += 4 * stride) {
+        uint2 k[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) k[u] = keys[e + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t b = f((uint64_t)k[u].x | ((uint64_t)k[u].y << 32));
+            if (b < nbins) atomicAdd(&my[b], 1u);
+        }
+    }
+    for (; e < n; e += stride) {
+        const uint2 k = keys[e];
+        const uint32_t b = f((uint64_t)k.x | ((uint64_t)k.y << 32));
+        if (b < nbins) atomicAdd(&my[b], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = tid; b < nbins; b += kThreads) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) s += s_h[w * 256 + b];
+        if (s) atomicAdd(&d_hist[b], s);
+    }
+}
+
+hipError_t launch_histogram_u64(const uint64_t* keys, uint32_t n, uint32_t* d_hist,
+                                uint32_t nbins, const AnyF64& f, hipStream_t stream) {
+    const size_t want = ((size_t)n + 4 * kThreads - 1) / (4 * kThreads);
+    const uint32_t grid = (uint32_t)std::max<size_t>(1, std::min<size_t>(want, 2048));
+    hipLaunchKernelGGL(hist_u64_kernel, dim3(grid), dim3(kThreads), 0, stream,
+                       reinterpret_cast<const uint2*>(keys), n, nbins, d_hist, f);
+    return hipGetLastError();
+}
+
 static uint32_t hist_grid(uint32_t n, uint32_t max_grid = kHistGrid) {
     const uint32_t full = n / kHistTile;
     return full < max_grid ? (full ? full : 1u) : max_grid;

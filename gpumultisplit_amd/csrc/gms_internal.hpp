@@ -51,6 +51,8 @@ hipError_t launch_histogram_u32(const uint32_t* kin, uint32_t n, uint32_t* d_his
                                 gms_bucket_desc d, hipStream_t stream);
 hipError_t launch_histogram_even_f32(const float* x, uint32_t n, int32_t* d_hist, uint32_t nbins,
                                      float lower, float inv_delta, hipStream_t stream);
+hipError_t launch_histogram_u64(const uint64_t* keys, uint32_t n, uint32_t* d_hist,
+                                uint32_t nbins, const AnyF64& f, hipStream_t stream);
 hipError_t launch_histogram_range_f32(const float* x, uint32_t n, int32_t* d_hist,
                                       uint32_t num_levels, const float* d_levels,
                                       hipStream_t stream);

@@ -308,6 +308,13 @@ gms_status gms_histogram_even_f32(const float *d_samples, uint32_t n, int32_t *d
                                   uint32_t num_bins, float lower, float upper,
                                   float inv_delta, void *stream);
 
+/* The same for 32- or 64-bit keys (key_bytes 4 or 8; a 64-bit array 8-byte
+ * aligned) with a gms_bucket_desc64 identifier: the reference's histogram
+ * kernels templated on KeyT (multisplit2_histograms.cuh:30-392). */
+gms_status gms_histogram_wide(const void *d_keys, uint32_t n, uint32_t key_bytes,
+                              uint32_t *d_hist, uint32_t num_bins, gms_bucket_desc64 desc,
+                              void *stream);
+
 /* Histogram with custom bin edges: levels (device, num_levels ascending
  * values, 2 <= num_levels <= 257) define num_levels - 1 bins [levels[i],
  * levels[i+1]); samples outside [levels[0], levels[num_levels-1]) and NaN are

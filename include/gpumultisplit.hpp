@@ -20,9 +20,9 @@
 //    specialising gms::bucket_traits<T> with
 //      static gms_bucket_desc desc(const T&, uint32_t num_buckets);
 //    mapping it to one of the built-in kinds.
-//  * KeyT and ValueT of multisplits, sorts and compactions may be 32- or
-//    64-bit (any 8-byte operand takes the 64-bit kernels); histograms take
-//    32-bit keys.  A 64-bit identifier parameter (a delta above 2^32) needs
+//  * KeyT and ValueT of multisplits, sorts, compactions and functor
+//    histograms may be 32- or 64-bit (any 8-byte operand takes the 64-bit
+//    kernels).  A 64-bit identifier parameter (a delta above 2^32) needs
 //    gms::delta_func64.  Sorts of 64-bit keys or values are
 //    allocated through the typed forms, e.g.
 //    multisplit_sort_8bit_allocate_pairs<uint64_t, uint64_t>(n, ctx).
@@ -479,10 +479,16 @@ template <typename KeyT, typename bucket_t>
 inline void multisplit_histogram(const KeyT* d_key_in, uint32_t num_elements, uint32_t* d_hist,
                                  uint32_t num_bins, bucket_t bucket_identifier,
                                  void* stream = nullptr) {
-    gms::require_32bit<KeyT>("key");
-    gms::check(gms_histogram_u32(reinterpret_cast<const uint32_t*>(d_key_in), num_elements, d_hist,
-                                 num_bins, gms::desc_of(bucket_identifier, num_bins), stream),
-               "multisplit_histogram");
+    if constexpr (gms::elem_bytes<KeyT>() == 8) {
+        gms::check(gms_histogram_wide(d_key_in, num_elements, 8, d_hist, num_bins,
+                                      gms::desc64_of(bucket_identifier, num_bins), stream),
+                   "multisplit_histogram");
+    } else {
+        gms::check(gms_histogram_u32(reinterpret_cast<const uint32_t*>(d_key_in), num_elements,
+                                     d_hist, num_bins, gms::desc_of(bucket_identifier, num_bins),
+                                     stream),
+                   "multisplit_histogram");
+    }
 }
 
 #endif  // GPUMULTISPLIT_HPP_

@@ -218,6 +218,16 @@ static void wide_cases(uint32_t n) {
     EXPECT(offs == wo, "multisplit_key_value<u64,u64> h_offsets n=%u", n);
     multisplit_release_memory(ctx);
 
+    // The same 64-bit identifier through the functor histogram: the counts are
+    // the differences of the oracle's bucket starts.
+    DevBuf dh(B);
+    dh.put(std::vector<uint32_t>(B, 0u));
+    multisplit_histogram(reinterpret_cast<const uint64_t*>(dk.p), n, dh.p, B,
+                         gms::delta_func64(delta));
+    std::vector<uint32_t> wh(B);
+    for (uint32_t b = 0; b < B; ++b) wh[b] = (b + 1 < B ? wo[b + 1] : n) - wo[b];
+    EXPECT(dh.get(B) == wh, "multisplit_histogram<u64> n=%u", n);
+
     DevBuf dv32(n), dvo32(n);
     dv32.put(v32);
     ms_sort_context sc;

@@ -406,6 +406,33 @@ def test_emulated_multisplit_wide_matches_oracle(emu, case, kbytes, vbytes):
             np.testing.assert_array_equal(got[1], wv)
 
 
+def wide_counts(keys, B, kind, param, width):
+    """Per-bucket counts from the oracle's bucket starts (64- or 32-bit keys)."""
+    _, _, wo = oracle.multisplit_wide(keys, B, kind, param, None, width)
+    return np.diff(np.append(wo.astype(np.int64), keys.size)).astype(np.uint32)
+
+
+@pytest.mark.parametrize("case", range(len(WIDE_CASES)))
+@pytest.mark.parametrize("kbytes", [8, 4])
+def test_emulated_histogram_wide(emu, case, kbytes):
+    B, kind, param, width = WIDE_CASES[case]
+    if kbytes == 4 and param > 0xFFFFFFFF:
+        param = 1000
+    if kbytes == 4 and kind == gms.BUCKET_DIGIT:
+        param = 26
+    d = gms.gms_bucket_desc64(kind, width, param)
+    for n in (0, 1, 77, 4 * 1024 + 5, 3 * 4 * 1024 * 8 + 11):
+        keys = _wide_keys(n, seed=n + case, kbytes=kbytes)
+        h = np.full(B, 3, np.uint32)  # the histogram accumulates
+        assert emu.gms_histogram_wide(p(keys), n, kbytes, p(h), B, d, None) == 0
+        np.testing.assert_array_equal(h - 3, wide_counts(keys, B, kind, param, width))
+    invalid_arg, unsupported_buckets = 1, 2
+    h, k = np.zeros(4, np.uint32), np.zeros(4, np.uint64)
+    assert emu.gms_histogram_wide(p(k), 4, 2, p(h), 4, d, None) == invalid_arg
+    assert emu.gms_histogram_wide(p(k), 4, 8, p(h), 257, gms.gms_bucket_desc64(0, 0, 0),
+                                  None) == unsupported_buckets
+
+
 @pytest.mark.parametrize("kbytes,vbytes", [(8, 0), (8, 8), (8, 4), (4, 8)])
 def test_emulated_sort_wide(emu, kbytes, vbytes):
     n = 2 * 256 * 12 + 9

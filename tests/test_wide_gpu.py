@@ -216,6 +216,23 @@ def test_multisplit_wide_more_than_256_buckets(dev, kbytes, vbytes):
                 np.testing.assert_array_equal(got[1], wv)
 
 
+@pytest.mark.parametrize("case", range(len(IDENTS)))
+def test_histogram_wide(dev, case):
+    # 64-bit keys through the functor histogram; the counts are the
+    # differences of the oracle's bucket starts, and the histogram accumulates.
+    import torch
+    B, kind, param, width = IDENTS[case]
+    f = _Ident64(kind, param, width)
+    for n in (0, 1, 3073, 1_000_003, (1 << 24) + 5):
+        banner(f"histogram wide B={B} kind={kind} n={n}")
+        keys = keys_of(n, seed=n + 7 * case, kbytes=8)
+        hist = torch.full((B,), 3, dtype=torch.int32, device=dev)
+        gms.histogram(to_dev_any(keys, dev) if n else out_like(keys, dev), n, hist, B, f)
+        _, _, wo = oracle.multisplit_wide(keys, B, kind, param, None, width)
+        want = np.diff(np.append(wo.astype(np.int64), n))
+        np.testing.assert_array_equal(hist.cpu().numpy().astype(np.int64) - 3, want)
+
+
 @pytest.mark.parametrize("kbytes,vbytes", [(8, 0), (8, 8), (4, 8)])
 def test_compaction_wide(dev, kbytes, vbytes):
     import torch
