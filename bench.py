@@ -36,7 +36,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--workload", default="ms_kv_256",
                    help="ms_{kv,ko}_<B>[_<dist>][_n<log2n>] | sort{7,8}_{kv,ko} | "
-                        "hist256 | histrange256 | compact{50,5} | copy | wide_{kv,ko}_256 | wide_sort8_kv | "
+                        "hist256 | histrange256 | compact{50,5} | copy | wide_{kv,ko}_256 | wide_sort8_kv | wide_hist256 | "
                         "variant_sweep")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-stage-events", action="store_true")
@@ -232,7 +232,8 @@ def bench_aux(args, torch, gms, W, dev):
     wide_kv_256 / wide_ko_256: 256-bucket range multisplit of 2^27 uint64 keys
       with (or without) uint64 values;
     wide_sort8_kv: 8-bit-digit radix sort of 2^25 uint64 keys with uint64
-      values (8 passes)."""
+      values (8 passes);
+    wide_hist256: 256-bin range histogram (top 8 bits) of 2^27 uint64 keys."""
     import numpy as np
     w = args.workload
     stream = torch.cuda.current_stream(dev).cuda_stream
@@ -262,6 +263,8 @@ def bench_aux(args, torch, gms, W, dev):
     elif w == "wide_sort8_kv":
         n, bpk = 1 << 25, 8 * 40
         metric = f"key-value radix sort (8-bit digits), {n} uint64 keys + uint64 values"
+    elif w == "wide_hist256":
+        n, bpk, metric = 1 << 27, 8, "256-bin range histogram, 134217728 uint64 keys"
     else:
         raise SystemExit(f"unknown workload {w!r}")
     wide = w.startswith("wide")
@@ -335,6 +338,14 @@ def bench_aux(args, torch, gms, W, dev):
                                  vout.data_ptr(), n, 8, 8, stream)
             if st != 0:
                 raise gms.GmsError(st, "gms_sort_wide")
+    elif w == "wide_hist256":
+        h = torch.zeros(256, dtype=torch.int32, device=dev)
+        desc = gms.range_bucket(256).desc64()
+
+        def step():
+            st = L.gms_histogram_wide(kin.data_ptr(), n, 8, h.data_ptr(), 256, desc, stream)
+            if st != 0:
+                raise gms.GmsError(st, "gms_histogram_wide")
     else:
         def step():
             gms.copy_u32(kin, kout, n)

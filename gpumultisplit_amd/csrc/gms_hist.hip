@@ -20,16 +20,20 @@ constexpr uint32_t kRangeGrid = 2048;
 
 // bins_of(x, b) maps a register array of words to bins (>= nbins: not
 // counted); it sees a whole tile row at once so that a kernel can issue the
-// lookups of all its items before it waits for any.
-template <bool VEC, class BinsOf>
+// lookups of all its items before it waits for any.  Elements are W words
+// (1, or 2 for 64-bit keys): a 16-byte vector of a striped row holds whole
+// elements, and bins_of reports an element's bin at its first word and an
+// out-of-range bin at its others.  n counts elements.
+template <bool VEC, int W = 1, class BinsOf>
 __device__ __forceinline__ void hist_body(const uint32_t* __restrict__ p, uint32_t n, uint32_t nbins,
                                           uint32_t* __restrict__ d_hist, BinsOf bins_of) {
+    static_assert(W == 1 || W == 2, "elements of 1 or 2 words");
     __shared__ uint32_t s_h[kWaves * 256];
     const uint32_t tid = threadIdx.x, wid = tid >> 6;
     for (uint32_t i = tid; i < kWaves * 256u; i += kThreads) s_h[i] = 0;
     __syncthreads();
     uint32_t* my = s_h + wid * 256u;
-    const uint32_t nfull = n / kHistTile;
+    const uint32_t nfull = (uint32_t)((size_t)n * W / kHistTile);
     uint32_t t = blockIdx.x;
     uint32_t x[kHistItems];
     if (t < nfull) load_striped4_full<kHistItems, VEC>(p, (size_t)t * kHistTile, x);
@@ -47,10 +51,14 @@ __device__ __forceinline__ void hist_body(const uint32_t* __restrict__ p, uint32
         t = tn;
     }
     if (blockIdx.x == 0) {
-        for (uint32_t i = nfull * kHistTile + tid; i < n; i += kThreads) {
-            uint32_t xt[1] = {p[i]}, bt[1];
+        for (uint32_t e = nfull * (kHistTile / W) + tid; e < n; e += kThreads) {
+            uint32_t xt[W], bt[W];
+#pragma unroll
+            for (int w = 0; w < W; ++w) xt[w] = p[(size_t)e * W + w];
             bins_of(xt, bt);
-            if (bt[0] < nbins) atomicAdd(&my[bt[0]], 1u);
+#pragma unroll
+            for (int w = 0; w < W; ++w)
+                if (bt[w] < nbins) atomicAdd(&my[bt[w]], 1u);
         }
     }
     __syncthreads();
@@ -218,50 +226,20 @@ __global__ __launch_bounds__(kThreads) void hist_range_kernel(const uint32_t* __
     });
 }
 
-// 64-bit keys through a 64-bit identifier: each thread counts 4 keys per
-// step with 8-byte loads (u64 arrays are 8-byte aligned) into its wave's LDS
-// sub-histogram; a resident grid walks the array.
-__global__ __launch_bounds__(kThreads) void hist_u64_kernel(const uint2* __restrict__ keys,
+// 64-bit keys through a 64-bit identifier: the same prefetching resident
+// grid over the key array viewed as words, each (low, high) word pair one key.
+template <bool VEC>
+__global__ __launch_bounds__(kThreads) void hist_u64_kernel(const uint32_t* __restrict__ words,
                                                             uint32_t n, uint32_t nbins,
                                                             uint32_t* __restrict__ d_hist, AnyF64 f) {
-    __shared__ uint32_t s_h[kWaves * 256];
-    const uint32_t tid = threadIdx.x, wid = tid >> 6;
-    for (uint32_t i = tid; i < kWaves * 256u; i += kThreads) s_h[i] = 0;
-    __syncthreads();
-    uint32_t* my = s_h + wid * 256u;
-    const size_t stride = (size_t)gridDim.x * kThreads;
-    size_t e = (size_t)blockIdx.x * kThreads + tid;
-    for (; e + 3 * stride < n; e += 4 * stride) {
-        uint2 k[4];
+    hist_body<VEC, 2>(words, n, nbins, d_hist, [&](auto& x, auto& b) {
+        constexpr int N = (int)(sizeof(x) / sizeof(x[0]));
 #pragma unroll
-        for (int u = 0; u < 4; ++u) k[u] = keys[e + u * stride];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint32_t b = f((uint64_t)k[u].x | ((uint64_t)k[u].y << 32));
-            if (b < nbins) atomicAdd(&my[b], 1u);
+        for (int k = 0; k < N; k += 2) {
+            b[k] = f((uint64_t)x[k] | ((uint64_t)x[k + 1] << 32));
+            b[k + 1] = 0xFFFFFFFFu;
         }
-    }
-    for (; e < n; e += stride) {
-        const uint2 k = keys[e];
-        const uint32_t b = f((uint64_t)k.x | ((uint64_t)k.y << 32));
-        if (b < nbins) atomicAdd(&my[b], 1u);
-    }
-    __syncthreads();
-    for (uint32_t b = tid; b < nbins; b += kThreads) {
-        uint32_t s = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) s += s_h[w * 256 + b];
-        if (s) atomicAdd(&d_hist[b], s);
-    }
-}
-
-hipError_t launch_histogram_u64(const uint64_t* keys, uint32_t n, uint32_t* d_hist,
-                                uint32_t nbins, const AnyF64& f, hipStream_t stream) {
-    const size_t want = ((size_t)n + 4 * kThreads - 1) / (4 * kThreads);
-    const uint32_t grid = (uint32_t)std::max<size_t>(1, std::min<size_t>(want, 2048));
-    hipLaunchKernelGGL(hist_u64_kernel, dim3(grid), dim3(kThreads), 0, stream,
-                       reinterpret_cast<const uint2*>(keys), n, nbins, d_hist, f);
-    return hipGetLastError();
+    });
 }
 
 static uint32_t hist_grid(uint32_t n, uint32_t max_grid = kHistGrid) {
@@ -270,6 +248,20 @@ static uint32_t hist_grid(uint32_t n, uint32_t max_grid = kHistGrid) {
 }
 
 static bool aligned16(const void* p) { return (((uintptr_t)p) & 15u) == 0; }
+
+hipError_t launch_histogram_u64(const uint64_t* keys, uint32_t n, uint32_t* d_hist,
+                                uint32_t nbins, const AnyF64& f, hipStream_t stream) {
+    const size_t full = (size_t)n * 2 / kHistTile;
+    const uint32_t grid = full < kHistGrid ? (full ? (uint32_t)full : 1u) : kHistGrid;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(keys);
+    if (aligned16(keys))
+        hipLaunchKernelGGL((hist_u64_kernel<true>), dim3(grid), dim3(kThreads), 0, stream, w, n,
+                           nbins, d_hist, f);
+    else
+        hipLaunchKernelGGL((hist_u64_kernel<false>), dim3(grid), dim3(kThreads), 0, stream, w, n,
+                           nbins, d_hist, f);
+    return hipGetLastError();
+}
 
 template <class F>
 static hipError_t launch_hist(const uint32_t* kin, uint32_t n, uint32_t* d_hist, uint32_t nbins,

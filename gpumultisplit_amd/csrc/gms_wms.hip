@@ -22,7 +22,6 @@ namespace gms {
 
 constexpr uint32_t kWmsItems = 16;                       // keys per lane
 constexpr uint32_t kWmsSub = 64u * kWmsItems;            // keys per wave subproblem
-constexpr uint32_t kWmsBlock = kWmsSub * (kThreads / 64);  // keys per workgroup
 
 uint32_t wms_subproblem_keys() { return kWmsSub; }
 

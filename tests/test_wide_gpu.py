@@ -223,11 +223,13 @@ def test_histogram_wide(dev, case):
     import torch
     B, kind, param, width = IDENTS[case]
     f = _Ident64(kind, param, width)
-    for n in (0, 1, 3073, 1_000_003, (1 << 24) + 5):
-        banner(f"histogram wide B={B} kind={kind} n={n}")
+    for n, off in ((0, 0), (1, 0), (3073, 1), (1_000_003, 0), (1_000_003, 1), ((1 << 24) + 5, 0)):
+        banner(f"histogram wide B={B} kind={kind} n={n} offset={off}")
         keys = keys_of(n, seed=n + 7 * case, kbytes=8)
         hist = torch.full((B,), 3, dtype=torch.int32, device=dev)
-        gms.histogram(to_dev_any(keys, dev) if n else out_like(keys, dev), n, hist, B, f)
+        # off = 1: an 8- but not 16-byte-aligned key array
+        kin = to_dev_any(np.concatenate([np.zeros(off, np.uint64), keys]), dev)[off:]
+        gms.histogram(kin if n else out_like(keys, dev), n, hist, B, f)
         _, _, wo = oracle.multisplit_wide(keys, B, kind, param, None, width)
         want = np.diff(np.append(wo.astype(np.int64), n))
         np.testing.assert_array_equal(hist.cpu().numpy().astype(np.int64) - 3, want)
