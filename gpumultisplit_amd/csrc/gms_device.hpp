@@ -218,6 +218,32 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 #endif
 }
 
+// Adds one to cnt[d] for every lane of the wave with `valid` set (cnt in LDS,
+// private to the wave).  Keys that share a bucket across the wave (sorted
+// runs, one dominant bucket) would make up to 64 lanes add to one address,
+// which the LDS serialises: the lanes holding the id of the lowest pending
+// lane are found with a ballot and added by that lane in one operation, for
+// two such ids; the rest add one each.  On evenly spread ids the match is
+// usually that lane alone, which then adds with the rest.
+__device__ __forceinline__ void wave_count_add(uint32_t* cnt, uint32_t d, bool valid) {
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t rest = GMS_BALLOT(valid);
+    bool pending = valid;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        if (rest == 0) break;  // the same in every lane
+        const int first = __builtin_ctzll(rest);
+        const uint32_t dl = (uint32_t)GMS_READLANE((int)d, first);
+        const uint64_t m = GMS_BALLOT(pending && d == dl);
+        if (__builtin_popcountll(m) > 1) {
+            if (lane == (uint32_t)first) atomicAdd(&cnt[dl], (uint32_t)__builtin_popcountll(m));
+            pending = pending && d != dl;
+        }
+        rest &= ~m;
+    }
+    if (pending) atomicAdd(&cnt[d], 1u);
+}
+
 // Exclusive scan across a block of NW waves; `total` receives the block sum.
 // s_w must hold NW words; the call contains one __syncthreads and callers
 // must separate consecutive uses of the same s_w by a barrier.

@@ -17,6 +17,7 @@ constexpr int kHistItems = 16;
 constexpr uint32_t kHistTile = kThreads * kHistItems;
 constexpr uint32_t kHistGrid = 512;
 constexpr uint32_t kRangeGrid = 2048;
+constexpr int kHistCopies = 8;
 
 // bins_of(x, b) maps a register array of words to bins (>= nbins: not
 // counted); it sees a whole tile row at once so that a kernel can issue the
@@ -24,15 +25,25 @@ constexpr uint32_t kRangeGrid = 2048;
 // (1, or 2 for 64-bit keys): a 16-byte vector of a striped row holds whole
 // elements, and bins_of reports an element's bin at its first word and an
 // out-of-range bin at its others.  n counts elements.
-template <bool VEC, int W = 1, class BinsOf>
+//
+// Every wave counts into C copies of its sub-histogram, lane l into copy
+// l % C, with rows of 257 words so that the copies of one bin lie in
+// different banks.  Keys that share a bin across a wave (sorted runs, one
+// dominant bin) then add to C addresses instead of one, which the LDS
+// serialises per address; evenly spread keys see the same bank spread as
+// with one copy.
+template <bool VEC, int W = 1, int C = kHistCopies, class BinsOf>
 __device__ __forceinline__ void hist_body(const uint32_t* __restrict__ p, uint32_t n, uint32_t nbins,
                                           uint32_t* __restrict__ d_hist, BinsOf bins_of) {
     static_assert(W == 1 || W == 2, "elements of 1 or 2 words");
-    __shared__ uint32_t s_h[kWaves * 256];
+    static_assert(C >= 1 && C <= 64 && (C & (C - 1)) == 0, "copies: a power of two");
+    constexpr uint32_t ROW = C > 1 ? 257u : 256u;
+    constexpr uint32_t NH = (uint32_t)(kWaves * C);
+    __shared__ uint32_t s_h[NH * ROW];
     const uint32_t tid = threadIdx.x, wid = tid >> 6;
-    for (uint32_t i = tid; i < kWaves * 256u; i += kThreads) s_h[i] = 0;
+    for (uint32_t i = tid; i < NH * ROW; i += kThreads) s_h[i] = 0;
     __syncthreads();
-    uint32_t* my = s_h + wid * 256u;
+    uint32_t* my = s_h + (wid * (uint32_t)C + (tid & (uint32_t)(C - 1))) * ROW;
     const uint32_t nfull = (uint32_t)((size_t)n * W / kHistTile);
     uint32_t t = blockIdx.x;
     uint32_t x[kHistItems];
@@ -51,10 +62,10 @@ __device__ __forceinline__ void hist_body(const uint32_t* __restrict__ p, uint32
         t = tn;
     }
     if (blockIdx.x == 0) {
-        for (uint32_t e = nfull * (kHistTile / W) + tid; e < n; e += kThreads) {
+        for (size_t e = (size_t)nfull * (kHistTile / W) + tid; e < n; e += kThreads) {
             uint32_t xt[W], bt[W];
 #pragma unroll
-            for (int w = 0; w < W; ++w) xt[w] = p[(size_t)e * W + w];
+            for (int w = 0; w < W; ++w) xt[w] = p[e * W + w];
             bins_of(xt, bt);
 #pragma unroll
             for (int w = 0; w < W; ++w)
@@ -65,7 +76,7 @@ __device__ __forceinline__ void hist_body(const uint32_t* __restrict__ p, uint32
     for (uint32_t b = tid; b < nbins; b += kThreads) {
         uint32_t s = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) s += s_h[w * 256 + b];
+        for (uint32_t q = 0; q < NH; ++q) s += s_h[q * ROW + b];
         if (s) atomicAdd(&d_hist[b], s);
     }
 }
@@ -200,7 +211,9 @@ __global__ __launch_bounds__(kThreads) void hist_range_kernel(const uint32_t* __
     for (uint32_t i = 0; i < kCellsPerThread; i += 4)
         *reinterpret_cast<uint4*>(mine + i) = make_uint4(c[i], c[i + 1], c[i + 2], c[i + 3]);
     // (hist_body's first barrier orders the table before any lookup)
-    hist_body<VEC>(xbits, n, nb, d_hist, [&](auto& x, auto& b) {
+    // One sub-histogram copy per wave: the cell table already takes most of
+    // a workgroup's share of LDS at this grid (8 workgroups per CU).
+    hist_body<VEC, 1, 1>(xbits, n, nb, d_hist, [&](auto& x, auto& b) {
         constexpr int N = (int)(sizeof(x) / sizeof(x[0]));
         T v[N];
         uint32_t cnt[N];

@@ -134,7 +134,7 @@ __device__ __forceinline__ void prescan_tile(const uint32_t (&x)[K], uint32_t nv
             const uint32_t w4 = 4u * ((uint32_t)j * kThreads + tid);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if (FULL || w4 + i < nvalid) atomicAdd(&my[f(x[4 * j + i])], 1u);
+                wave_count_add(my, f(x[4 * j + i]), FULL || w4 + i < nvalid);
         }
     }
     __syncthreads();

@@ -86,8 +86,7 @@ __global__ __launch_bounds__(kThreads) void ms_wide_prescan_kernel(const uint32_
         const uint32_t e4 = (4u * ((uint32_t)j * kThreads + tid)) / KW;  // first element of the stripe
 #pragma unroll
         for (int q = 0; q < 4 / KW; ++q)
-            if (e4 + (uint32_t)q < nvalid)
-                atomicAdd(&my[f(key_word_of<KW>(x + 4 * j, q))], 1u);
+            wave_count_add(my, f(key_word_of<KW>(x + 4 * j, q)), e4 + (uint32_t)q < nvalid);
     }
     __syncthreads();
     for (uint32_t b = tid; b < (uint32_t)B; b += kThreads) {

@@ -124,6 +124,46 @@ def test_histogram_unaligned_and_tails(dev):
         np.testing.assert_array_equal(from_dev(he).view(np.int32), oracle.histogram_even_f32(x, 64, 0.0, inv))
 
 
+def skewed_keys(kind, n, seed):
+    """Keys whose bins coincide across whole waves: sorted (long runs), one
+    bucket holding 90 % of shuffled keys, and every key equal."""
+    g = W.rng(seed)
+    k = g.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    if kind == "sorted":
+        return np.sort(k)
+    if kind == "dominant":
+        k[g.random(n) < 0.9] = np.uint32(0x9ABCDEF0)
+        return k
+    return np.full(n, 0x12345678, np.uint32)
+
+
+@pytest.mark.parametrize("kind", ["sorted", "dominant", "equal"])
+def test_skewed_keys_histogram_and_multisplit(dev, kind):
+    import torch
+    # Lanes of a wave that share a bin are counted together (histogram copies,
+    # ballot-combined prescan counts); the counts must stay exact.
+    for n in (4096 * 37 + 11, (1 << 22) + 3):
+        keys = skewed_keys(kind, n, seed=n)
+        d = to_dev(keys, dev)
+        for bins in (16, 256):
+            h = torch.zeros(bins, dtype=torch.int32, device=dev)
+            gms.histogram(d, n, h, bins, gms.range_bucket(bins))
+            np.testing.assert_array_equal(from_dev(h), oracle.histogram(keys, bins, oracle.RANGE))
+        for B in (16, 256):
+            vals = np.arange(n, dtype=np.uint32)
+            kout, vout = torch.empty_like(d), torch.empty_like(d)
+            offs = np.zeros(B, np.uint32)
+            ctx = gms.multisplit_context(B)
+            gms.multisplit_allocate_key_value(n, ctx)
+            gms.multisplit_key_value(d, to_dev(vals, dev), kout, vout, n, ctx, gms.range_bucket(B),
+                                     offs)
+            gms.multisplit_release_memory(ctx)
+            wk, wv, wo = oracle.multisplit(keys, B, oracle.RANGE, 0, vals)
+            np.testing.assert_array_equal(from_dev(kout), wk)
+            np.testing.assert_array_equal(from_dev(vout), wv)
+            np.testing.assert_array_equal(offs, wo)
+
+
 def test_histogram_256_low_byte_accumulates(dev):
     import torch
     # 256-bin histogram of the low 8 bits, added twice into the same array
