@@ -562,7 +562,7 @@ def main():
         "dtype": "u32",
         "data": f"synthetic ({cfg['dist']} keys, values = original index)",
         "config": {"workload": args.workload, "n": n, "buckets": B, "key_value": kv,
-                   "bucket_identifier": "range", "tile_keys": gms.lib().gms_subproblem_size(B, int(kv))},
+                   "bucket_identifier": "range", "tile_keys": gms.lib().gms_tile_keys(B, int(kv), n)},
         "call_hbm_fraction": round(gkeys * bpk_call / HBM_PEAK_GBS, 4),
         "roofline": roofline,
         "stages": stages,

@@ -55,8 +55,11 @@ bool ranges_overlap(const void* a, size_t an, const void* b, size_t bn) {
 size_t matrix_elems(uint32_t n, uint32_t num_buckets) {
     if (num_buckets <= 1 || n == 0) return 0;
     const int lb = ceil_log2_u32(num_buckets);
-    const size_t t_ko = ((size_t)n + tile_keys(lb, false, 0) - 1) / tile_keys(lb, false, 0);
-    const size_t t_kv = ((size_t)n + tile_keys(lb, true, 0) - 1) / tile_keys(lb, true, 0);
+    // the shortest of the short- and long-input tiles of each class
+    const uint32_t tko = std::min(tile_keys(lb, false, 0), tile_keys(lb, false, kLongTileMin));
+    const uint32_t tkv = std::min(tile_keys(lb, true, 0), tile_keys(lb, true, kLongTileMin));
+    const size_t t_ko = ((size_t)n + tko - 1) / tko;
+    const size_t t_kv = ((size_t)n + tkv - 1) / tkv;
     // the shortest tile of the 64-bit kernels (gms_multisplit_wide, gms_sort_wide)
     const size_t t_w = ((size_t)n + wide_tile_elems(2, 2) - 1) / wide_tile_elems(2, 2);
     // one column per wave subproblem of the wave-level variant (up to 64 buckets)
