@@ -218,6 +218,20 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 #endif
 }
 
+// Mask of the lanes of the wave whose LOG_B-bit id equals this lane's, among
+// the lanes with `valid` set: one ballot per id bit.
+template <int LOG_B>
+__device__ __forceinline__ uint64_t wave_peers(uint32_t d, bool valid) {
+    uint64_t peers = GMS_BALLOT(valid);
+#pragma unroll
+    for (int i = 0; i < LOG_B; ++i) {
+        const bool bit = (d >> i) & 1u;
+        const uint64_t m = GMS_BALLOT(valid && bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
 // Adds one to cnt[d] for every lane of the wave with `valid` set (cnt in LDS,
 // private to the wave).  Keys that share a bucket across the wave (sorted
 // runs, one dominant bucket) would make up to 64 lanes add to one address,

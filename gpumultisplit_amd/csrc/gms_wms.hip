@@ -25,19 +25,6 @@ constexpr uint32_t kWmsSub = 64u * kWmsItems;            // keys per wave subpro
 
 uint32_t wms_subproblem_keys() { return kWmsSub; }
 
-// Mask of the lanes whose bucket equals this lane's (among `valid` lanes).
-template <int LOG_B>
-__device__ __forceinline__ uint64_t wms_peers(uint32_t d, bool valid) {
-    uint64_t peers = GMS_BALLOT(valid);
-#pragma unroll
-    for (int i = 0; i < LOG_B; ++i) {
-        const bool bit = (d >> i) & 1u;
-        const uint64_t m = GMS_BALLOT(valid && bit);
-        peers &= bit ? m : ~m;
-    }
-    return peers;
-}
-
 template <int LOG_B, class F>
 __global__ __launch_bounds__(kThreads) void wms_prescan_kernel(const uint32_t* __restrict__ kin,
                                                                uint32_t n, uint32_t num_sub,
@@ -63,7 +50,7 @@ __global__ __launch_bounds__(kThreads) void wms_prescan_kernel(const uint32_t* _
     for (uint32_t k = 0; k < kWmsItems; ++k) {
         const bool valid = k * 64u + lane < nvalid;
         const uint32_t d = f(key[k]);
-        const uint64_t peers = wms_peers<LOG_B>(d, valid);
+        const uint64_t peers = wave_peers<LOG_B>(d, valid);
         if (valid && lanes_below(peers) == 0u) cnt[d] += (uint32_t)__builtin_popcountll(peers);
         GMS_WAVE_SYNC();
     }
@@ -116,7 +103,7 @@ __global__ __launch_bounds__(kThreads) void wms_postscan_kernel(
     for (uint32_t k = 0; k < kWmsItems; ++k) {
         const bool valid = k * 64u + lane < nvalid;
         const uint32_t d = f(key[k]);
-        const uint64_t peers = wms_peers<LOG_B>(d, valid);
+        const uint64_t peers = wave_peers<LOG_B>(d, valid);
         const uint32_t r = lanes_below(peers);
         if (valid) {
             uint32_t pos = run[d] + r;
