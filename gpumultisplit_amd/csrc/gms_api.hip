@@ -54,6 +54,19 @@ bool ranges_overlap(const void* a, size_t an, const void* b, size_t bn) {
 
 size_t matrix_elems(uint32_t n, uint32_t num_buckets) {
     if (num_buckets <= 1 || n == 0) return 0;
+    if (num_buckets > 256) {
+        // 64-bit elements, and 32-bit ones above kSinglePassBuckets, take
+        // passes of at most 256 buckets; 32-bit ones up to it take one pass.
+        size_t m = matrix_elems(n, 256);
+        if (num_buckets <= kSinglePassBuckets) {
+            const int lb = ceil_log2_u32(num_buckets);
+            const uint32_t tko = std::min(tile_keys(lb, false, 0), tile_keys(lb, false, kLongTileMin));
+            const uint32_t tkv = std::min(tile_keys(lb, true, 0), tile_keys(lb, true, kLongTileMin));
+            const size_t t = std::max(((size_t)n + tko - 1) / tko, ((size_t)n + tkv - 1) / tkv);
+            m = std::max(m, ((size_t)1 << lb) * t);
+        }
+        return m;
+    }
     const int lb = ceil_log2_u32(num_buckets);
     // the shortest of the short- and long-input tiles of each class
     const uint32_t tko = std::min(tile_keys(lb, false, 0), tile_keys(lb, false, kLongTileMin));
@@ -340,7 +353,7 @@ uint32_t gms_subproblem_size(uint32_t num_buckets, int key_value) {
 }
 
 uint32_t gms_tile_keys(uint32_t num_buckets, int key_value, uint32_t n) {
-    if (num_buckets < 1 || num_buckets > 256) return 0;
+    if (num_buckets < 1 || num_buckets > kSinglePassBuckets) return 0;
     return tile_keys(ceil_log2_u32(num_buckets), key_value != 0, n);
 }
 
@@ -360,8 +373,10 @@ gms_status gms_multisplit_allocate(gms_multisplit_ctx* ctx, uint32_t n_capacity,
     (void)key_value;  // the matrix is sized for both tile shapes
     if (!ctx) return GMS_ERR_INVALID_ARG;
     const bool large = ctx->num_buckets > 256;
-    gms_status st = scratch_alloc(ctx->s, matrix_elems(n_capacity, large ? 256u : ctx->num_buckets));
-    if (st == GMS_SUCCESS && large) st = large_scratch_alloc(ctx->large, n_capacity, ctx->num_buckets);
+    gms_status st = scratch_alloc(ctx->s, matrix_elems(n_capacity, ctx->num_buckets));
+    if (st == GMS_SUCCESS && large)
+        st = large_scratch_alloc(ctx->large, n_capacity, ctx->num_buckets,
+                                 ctx->num_buckets > kSinglePassBuckets);
     if (st != GMS_SUCCESS) {
         scratch_free(ctx->s);
         return st;
@@ -427,7 +442,7 @@ static gms_status run_ctx(gms_multisplit_ctx* ctx, const uint32_t* kin, const ui
                           uint32_t* d_offsets, hipStream_t s) {
     if (use_wms(ctx, n, vin != nullptr))
         return run_wms_multisplit(ctx->s, kin, vin, kout, vout, n, ctx->num_buckets, d, d_offsets, s);
-    if (ctx->num_buckets > 256)
+    if (ctx->num_buckets > kSinglePassBuckets)
         return run_large_multisplit(ctx->large, ctx->s, kin, vin, kout, vout, n, ctx->num_buckets,
                                     d, d_offsets, s);
     return run_multisplit(ctx->s, kin, vin, kout, vout, n, ctx->num_buckets, d, d_offsets, s,
@@ -590,7 +605,8 @@ gms_status gms_multisplit_prescan_matrix(gms_multisplit_ctx* ctx, const uint32_t
                                          void* stream) {
     if (!ctx) return GMS_ERR_INVALID_ARG;
     if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
-    if (ctx->num_buckets > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;  // no per-tile matrix there
+    // no per-tile matrix where LSD passes over digits of the id run
+    if (ctx->num_buckets > kSinglePassBuckets) return GMS_ERR_UNSUPPORTED_BUCKETS;
     if (n > ctx->capacity) return GMS_ERR_CAPACITY;
     gms_status st = check_desc(ctx->num_buckets, d);
     if (st != GMS_SUCCESS) return st;

@@ -126,6 +126,7 @@ __device__ __forceinline__ void batch_postscan_item(
     const uint32_t cntb = g1 - g0;
     const uint32_t g0l = g0 - I.obase;  // start inside this instance's output
     if (t == 0 && I.offs && tid < I.nb) I.offs[tid] = g0l;
+    const uint32_t g0a[1] = {g0l}, cnta[1] = {tid < (uint32_t)B ? cntb : 0u};
 
     uint32_t key[K], val[K];
     if (nvalid == TILE) {
@@ -135,7 +136,7 @@ __device__ __forceinline__ void batch_postscan_item(
 #pragma unroll
             for (int k = 0; k < K; ++k) val[k] = 0;
         }
-        postscan_rank<LOG_B, K, LEVEL_BITS, KV, true, ALIAS>(key, val, g0l, cntb, TILE, I.f, s_keys,
+        postscan_rank<LOG_B, K, LEVEL_BITS, KV, true, ALIAS>(key, val, g0a, cnta, TILE, I.f, s_keys,
                                                       s_vals, s_cnt, s_delta, s_bs, s_ws);
         postscan_write<K, KV, true>(TILE, I.n, I.kout, I.vout, I.f, s_keys, s_vals, s_delta);
     } else {
@@ -145,7 +146,7 @@ __device__ __forceinline__ void batch_postscan_item(
 #pragma unroll
             for (int k = 0; k < K; ++k) val[k] = 0;
         }
-        postscan_rank<LOG_B, K, LEVEL_BITS, KV, false, ALIAS>(key, val, g0l, cntb, nvalid, I.f, s_keys,
+        postscan_rank<LOG_B, K, LEVEL_BITS, KV, false, ALIAS>(key, val, g0a, cnta, nvalid, I.f, s_keys,
                                                        s_vals, s_cnt, s_delta, s_bs, s_ws);
         postscan_write<K, KV, false>(nvalid, I.n, I.kout, I.vout, I.f, s_keys, s_vals, s_delta);
     }

@@ -80,7 +80,10 @@ struct LargeScratch {
     uint32_t* offs = nullptr;  // num_buckets bucket starts (staging for host offsets)
     uint32_t cap = 0;
 };
-gms_status large_scratch_alloc(LargeScratch& L, uint32_t n_capacity, uint32_t num_buckets);
+// offs (num_buckets words) always; the two capacity-word pass temporaries
+// only with `temporaries` (bucket counts beyond one pass).
+gms_status large_scratch_alloc(LargeScratch& L, uint32_t n_capacity, uint32_t num_buckets,
+                               bool temporaries);
 void large_scratch_free(LargeScratch& L);
 gms_status run_large_multisplit(LargeScratch& L, Scratch& s, const uint32_t* kin,
                                 const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint32_t n,

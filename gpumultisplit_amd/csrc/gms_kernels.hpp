@@ -298,19 +298,30 @@ __device__ __forceinline__ void reload_blocked(uint32_t (&key)[K], uint32_t (&va
 // In-tile ranking of one tile: delta[b] = global start - local start of every
 // bucket run, then the stable LDS passes; leaves the tile in bucket order in
 // s_keys/s_vals.
-template <int LOG_B, int K, int LEVEL_BITS, bool KV, bool FULL, bool ALIAS, class F>
-__device__ __forceinline__ void postscan_rank(uint32_t (&key)[K], uint32_t (&val)[K], uint32_t g0,
-                                              uint32_t cntb, uint32_t nvalid, F f,
+// Thread tid holds buckets tid*R .. tid*R+R-1 (R = 1 up to 256 buckets): their
+// global starts g0[] and the tile's counts cnt[] (0 past the bucket count).
+template <int LOG_B, int K, int LEVEL_BITS, bool KV, bool FULL, bool ALIAS, int R, class F>
+__device__ __forceinline__ void postscan_rank(uint32_t (&key)[K], uint32_t (&val)[K],
+                                              const uint32_t (&g0)[R], const uint32_t (&cnt)[R],
+                                              uint32_t nvalid, F f,
                                               uint32_t* s_keys, uint32_t* s_vals,
                                               uint32_t* s_cnt, uint32_t* s_delta, uint32_t* s_bs,
                                               uint32_t* s_ws) {
     constexpr int B = 1 << LOG_B;
     constexpr int NLEV = (LOG_B + LEVEL_BITS - 1) / LEVEL_BITS;
+    static_assert(R * kThreads >= B, "R buckets per thread cover every bucket");
     const uint32_t tid = threadIdx.x;
     {
-        uint32_t tot;
-        const uint32_t ls = block_excl_scan<kWaves>(tid < (uint32_t)B ? cntb : 0u, s_bs, tot);
-        if (tid < (uint32_t)B) s_delta[tid] = g0 - ls;
+        uint32_t sum = 0, tot;
+#pragma unroll
+        for (int r = 0; r < R; ++r) sum += cnt[r];
+        uint32_t ls = block_excl_scan<kWaves>(sum, s_bs, tot);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t b = tid * (uint32_t)R + (uint32_t)r;
+            if (b < (uint32_t)B) s_delta[b] = g0[r] - ls;
+            ls += cnt[r];
+        }
     }
     constexpr int W0 = LOG_B < LEVEL_BITS ? LOG_B : LEVEL_BITS;
     rank_pass<0, W0, K, KV, FULL, ALIAS>(key, val, nvalid, f, s_keys, s_vals, s_cnt, s_ws);
@@ -384,13 +395,28 @@ __global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
     const size_t base = (size_t)t * TILE;
 
     // Scanned entry G[b][t] and the next one in scan order (n past the end):
-    // their difference is the tile's count in bucket b.  Loaded by every
-    // thread (threads past B reload bucket B-1 and ignore it).
-    const uint32_t bt = tid < (uint32_t)B ? tid : (uint32_t)B - 1u;
-    const size_t idx = (size_t)bt * num_tiles + t;
+    // their difference is the tile's count in bucket b.  Thread tid loads
+    // buckets tid*R .. tid*R+R-1 (threads past B reload bucket B-1 and ignore
+    // it).
+    constexpr int R = (B + kThreads - 1) / kThreads;
     const size_t mtot = (size_t)B * num_tiles;
-    const uint32_t g0 = G[idx];
-    const uint32_t g1raw = G[idx + 1 < mtot ? idx + 1 : idx];
+    uint32_t g0[R], g1raw[R];
+    bool last[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t b = tid * (uint32_t)R + (uint32_t)r;
+        const size_t idx = (size_t)(b < (uint32_t)B ? b : (uint32_t)B - 1u) * num_tiles + t;
+        last[r] = idx + 1 >= mtot;
+        g0[r] = G[idx];
+        g1raw[r] = G[last[r] ? idx : idx + 1];
+    }
+    auto tile_counts = [&](uint32_t (&cnt)[R]) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            cnt[r] = tid * (uint32_t)R + (uint32_t)r < (uint32_t)B
+                         ? (last[r] ? n : g1raw[r]) - g0[r]
+                         : 0u;
+    };
 
     uint32_t key[K], val[K];
     if (nvalid == TILE) {
@@ -400,8 +426,9 @@ __global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
 #pragma unroll
             for (int k = 0; k < K; ++k) val[k] = 0;
         }
-        const uint32_t cntb = (idx + 1 < mtot ? g1raw : n) - g0;
-        postscan_rank<LOG_B, K, LEVEL_BITS, KV, true, ALIAS>(key, val, g0, cntb, TILE, f, s_keys,
+        uint32_t cnt[R];
+        tile_counts(cnt);
+        postscan_rank<LOG_B, K, LEVEL_BITS, KV, true, ALIAS>(key, val, g0, cnt, TILE, f, s_keys,
                                                       s_vals, s_cnt, s_delta, s_bs, s_ws);
         postscan_write<K, KV, true>(TILE, n, kout, vout, f, s_keys, s_vals, s_delta);
     } else {
@@ -411,8 +438,9 @@ __global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
 #pragma unroll
             for (int k = 0; k < K; ++k) val[k] = 0;
         }
-        const uint32_t cntb = (idx + 1 < mtot ? g1raw : n) - g0;
-        postscan_rank<LOG_B, K, LEVEL_BITS, KV, false, ALIAS>(key, val, g0, cntb, nvalid, f, s_keys,
+        uint32_t cnt[R];
+        tile_counts(cnt);
+        postscan_rank<LOG_B, K, LEVEL_BITS, KV, false, ALIAS>(key, val, g0, cnt, nvalid, f, s_keys,
                                                        s_vals, s_cnt, s_delta, s_bs, s_ws);
         postscan_write<K, KV, false>(nvalid, n, kout, vout, f, s_keys, s_vals, s_delta);
     }

@@ -32,14 +32,17 @@ __global__ __launch_bounds__(kThreads) void bucket_starts_kernel(const uint32_t*
     }
 }
 
-gms_status large_scratch_alloc(LargeScratch& L, uint32_t n_capacity, uint32_t num_buckets) {
+gms_status large_scratch_alloc(LargeScratch& L, uint32_t n_capacity, uint32_t num_buckets,
+                               bool temporaries) {
     large_scratch_free(L);
     const size_t words = std::max<size_t>(n_capacity, 1);
-    if (hipMalloc((void**)&L.ktmp, words * sizeof(uint32_t)) != hipSuccess) L.ktmp = nullptr;
-    if (hipMalloc((void**)&L.vtmp, words * sizeof(uint32_t)) != hipSuccess) L.vtmp = nullptr;
+    if (temporaries) {
+        if (hipMalloc((void**)&L.ktmp, words * sizeof(uint32_t)) != hipSuccess) L.ktmp = nullptr;
+        if (hipMalloc((void**)&L.vtmp, words * sizeof(uint32_t)) != hipSuccess) L.vtmp = nullptr;
+    }
     if (hipMalloc((void**)&L.offs, (size_t)num_buckets * sizeof(uint32_t)) != hipSuccess)
         L.offs = nullptr;
-    if (!L.ktmp || !L.vtmp || !L.offs) {
+    if ((temporaries && (!L.ktmp || !L.vtmp)) || !L.offs) {
         large_scratch_free(L);
         return GMS_ERR_ALLOC;
     }
@@ -61,7 +64,7 @@ gms_status run_large_multisplit(LargeScratch& L, Scratch& s, const uint32_t* kin
                                 uint32_t B, gms_bucket_desc d, uint32_t* d_offsets,
                                 hipStream_t stream) {
     if (n == 0) return d_offsets ? hip_status(launch_fill(d_offsets, 0u, B, stream)) : GMS_SUCCESS;
-    if (n > L.cap) return GMS_ERR_CAPACITY;
+    if (n > L.cap || !L.ktmp || !L.vtmp) return GMS_ERR_CAPACITY;
     const AnyF base = AnyF::make(d.kind, d.param, d.width, B);
     const bool kv = vin != nullptr;
     const int bits = ceil_log2_u32(B);

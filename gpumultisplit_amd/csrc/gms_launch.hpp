@@ -65,6 +65,10 @@ hipError_t launch_postscan_t(const MsCall& c, const F& f) {
         case 15: return FN<7, true>(c, f);                             \
         case 16: return FN<8, false>(c, f);                            \
         case 17: return FN<8, true>(c, f);                             \
+        case 18: return FN<9, false>(c, f);                            \
+        case 19: return FN<9, true>(c, f);                             \
+        case 20: return FN<10, false>(c, f);                           \
+        case 21: return FN<10, true>(c, f);                            \
         default: return hipErrorInvalidValue;                          \
     }
 

@@ -160,8 +160,8 @@ hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t m, 
 
 hipError_t launch_gather_offsets(const uint32_t* G, uint32_t num_tiles, uint32_t num_buckets,
                                  uint32_t* out, hipStream_t stream) {
-    hipLaunchKernelGGL(gather_offsets_kernel, dim3(1), dim3(256), 0, stream, G, num_tiles,
-                       num_buckets, out);
+    hipLaunchKernelGGL(gather_offsets_kernel, dim3((num_buckets + 255) / 256), dim3(256), 0, stream,
+                       G, num_tiles, num_buckets, out);
     return hipGetLastError();
 }
 

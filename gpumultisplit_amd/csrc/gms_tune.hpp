@@ -50,6 +50,11 @@ constexpr bool tune_alias(int log_b, int items, bool kv, int level_bits) {
     return kv && cnt_row(1 << level_bits) <= items && !(log_b >= 7 && items < 28);
 }
 
+// Bucket counts up to this run in one prescan/scan/postscan pass (three LDS
+// ranking passes of at most 4 bits above 256); more take LSD passes over
+// 8-bit digits of the bucket id.
+constexpr uint32_t kSinglePassBuckets = 1024;
+
 // Keys per tile for an input of n keys.  The shortest tile, used to size the
 // histogram matrix for every n up to a capacity, is tile_keys(log_b, kv, 0).
 constexpr uint32_t tile_keys(int log_b, bool kv, uint32_t n) {

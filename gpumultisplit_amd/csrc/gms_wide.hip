@@ -114,7 +114,9 @@ __device__ __forceinline__ void wide_postscan_tile(
         pk[k] = (f(key_word_of<KW>(kx, k)) << 16) | (e0 + (uint32_t)k);
         unused[k] = 0;
     }
-    postscan_rank<LOG_B, K, LEVEL_BITS, false, FULL, false>(pk, unused, g0, cntb, nvalid,
+    constexpr uint32_t B = 1u << LOG_B;
+    const uint32_t g0a[1] = {g0}, cnta[1] = {tid < B ? cntb : 0u};
+    postscan_rank<LOG_B, K, LEVEL_BITS, false, FULL, false>(pk, unused, g0a, cnta, nvalid,
                                                             PackedBidF{}, s_keys, nullptr, s_u,
                                                             s_delta, s_bs, s_ws);
     // The ranking ended with a barrier: its counters in s_u are dead.

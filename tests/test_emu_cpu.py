@@ -295,11 +295,15 @@ def test_emulated_server_budget_admission(emu):
 
 
 @pytest.mark.parametrize("B,kind,param,width", [(257, oracle.RANGE, 0, 0), (1000, oracle.DELTA, 4294968, 0),
+                                                (512, oracle.DIGIT, 5, 9), (1024, oracle.RANGE, 0, 0),
+                                                (1025, oracle.RANGE, 0, 0),
                                                 (4096, oracle.DIGIT, 7, 12), (70000, oracle.IDENTITY, 0, 0),
                                                 (1 << 20, oracle.RANGE, 0, 0)])
 def test_emulated_more_than_256_buckets(emu, B, kind, param, width):
-    """Reduced-bit-sort path: ids, LSD passes over the id bits, gather."""
-    n = 7001
+    """Up to 1024 buckets: one pass with three LDS ranking passes; beyond:
+    LSD passes over 8-bit digits of the id, bucket starts read off the output.
+    Sizes span several tiles and a ragged last one."""
+    n = 2 * 6144 + 7001
     keys = W.uniform_keys(n, seed=B)
     if kind == oracle.IDENTITY:
         keys = keys % np.uint32(B + 100)
