@@ -96,11 +96,12 @@ gms_status gms_bucket_ids64_host(uint32_t num_buckets, gms_bucket_desc64 desc,
 typedef struct gms_multisplit_ctx gms_multisplit_ctx;
 
 /* multisplit_context(uint32_t n_buckets) (src/api/multisplit.cuh:29-47).
- * 1 <= num_buckets <= 2^20.  Up to 256 buckets the multisplit runs as
- * prescan -> scan -> postscan (the reference's API accepts powers of two
- * 2..256); above 256 as a stable radix sort of the bucket ids, the reduced-bit
- * sort the reference points to for more buckets (src/kernels/RBsort/
- * reduced_bit_sort.cuh:24-57, src/main/main_wms.cu:534-755). */
+ * 1 <= num_buckets <= 2^20.  Up to 1024 buckets the multisplit of 32-bit
+ * elements runs as one prescan -> scan -> postscan (the reference's API
+ * accepts powers of two 2..256); above that as a stable radix sort of the
+ * bucket ids, the reduced-bit sort the reference points to for more buckets
+ * (src/kernels/RBsort/reduced_bit_sort.cuh:24-57, src/main/main_wms.cu:
+ * 534-755). */
 gms_status gms_multisplit_create(uint32_t num_buckets, gms_multisplit_ctx **out);
 
 /* multisplit_allocate_key_only(n, ctx)  (src/api/multisplit.cuh:50-61) with
@@ -196,7 +197,7 @@ gms_status gms_multisplit_profile(gms_multisplit_ctx *ctx, double *ms_prescan,
                                   double *ms_scan, double *ms_postscan,
                                   uint32_t *calls);
 
-/* Stage-level check (up to 256 buckets): runs only the prescan on d_key_in and copies the per-tile
+/* Stage-level check (up to 1024 buckets): runs only the prescan on d_key_in and copies the per-tile
  * bucket counts, bucket-major (H[b * tiles + t]), to d_matrix (B * tiles u32,
  * tiles = ceil(n / gms_tile_keys(B, key_value, n))).  Same quantity the reference's
  * cpu_prescan_test checks (src/cpu_functions.h:55-103). */

@@ -207,7 +207,8 @@ def test_device_offsets_and_determinism(dev):
 def test_prescan_matrix_stage(dev):
     import torch
     for B, n, kv in ((256, 100_000, True), (8, 50_001, True), (3, 9000, True),
-                     (256, (1 << 26) + 5, False), (64, (1 << 26) + 3, True)):
+                     (256, (1 << 26) + 5, False), (64, (1 << 26) + 3, True),
+                     (1000, 70_001, True), (1024, 300_000, False)):
         ctx = gms.multisplit_context(B)
         gms.multisplit_allocate_key_value(n, ctx)
         keys = W.uniform_keys(n, seed=B)
