@@ -97,7 +97,8 @@ int main(int argc, char** argv) {
     const uint32_t tile_kv = gms_subproblem_size(256, 1);
     const uint32_t sizes[] = {0, 1, 2, 63, 64, 65, 1000, tile_kv - 1, tile_kv, tile_kv + 1,
                               3 * tile_kv + 17};
-    const uint32_t bs[] = {1, 2, 3, 4, 5, 8, 16, 17, 32, 33, 64, 100, 128, 129, 255, 256};
+    const uint32_t bs[] = {1, 2, 3, 4, 5, 8, 16, 17, 32, 33, 64, 100, 128, 129, 255, 256,
+                           300, 512, 1000, 1024};
     for (uint32_t B : bs) {
         for (uint32_t n : sizes) {
             if (quick && n > tile_kv + 1) continue;
