@@ -351,6 +351,16 @@ def test_plain_sweep_2e27_uniform(dev, B, kv):
 
 
 @pytest.mark.slow
+@pytest.mark.parametrize("n,dist", [(1 << 27, "uniform"), (1 << 25, "zipf"), (1 << 25, "onehot90")])
+@pytest.mark.parametrize("B", [512, 1000, 1024])
+@pytest.mark.parametrize("kv", [False, True])
+def test_one_pass_up_to_1024_buckets_properties(dev, n, dist, B, kv):
+    """257..1024 buckets run as one pass with three LDS ranking passes; full
+    sizes (long tiles from 2^26 keys) and skewed keys, by the same properties."""
+    _check_properties(dev, n, dist, B, kv)
+
+
+@pytest.mark.slow
 @pytest.mark.parametrize("B,kv,extra", [(128, False, 12345), (64, True, 777), (256, False, 1),
                                         (256, True, 6143)])
 def test_long_tiles_against_oracle(dev, B, kv, extra):

@@ -235,6 +235,24 @@ def test_histogram_wide(dev, case):
         np.testing.assert_array_equal(hist.cpu().numpy().astype(np.int64) - 3, want)
 
 
+@pytest.mark.slow
+def test_histogram_wide_full_size(dev):
+    # 2^28 + 5 uint64 keys (2 GiB) from torch's generator: the 256-bin range
+    # histogram is the count of each top byte, counted here by torch.
+    import torch
+    n = (1 << 28) + 5
+    banner(f"histogram wide full size n={n}")
+    g = torch.Generator(device=dev)
+    g.manual_seed(28)
+    k = torch.randint(-(1 << 62), 1 << 62, (n,), dtype=torch.int64, device=dev, generator=g) * 2
+    k[::7] = -1  # 2^64 - 1: the last bin
+    hist = torch.zeros(256, dtype=torch.int32, device=dev)
+    gms.histogram(k, n, hist, 256, gms.range_bucket(256))
+    want = torch.bincount((k >> 56) & 0xFF, minlength=256)
+    assert int(hist.to(torch.int64).sum()) == n
+    assert bool((hist.to(torch.int64) == want).all())
+
+
 @pytest.mark.parametrize("kbytes,vbytes", [(8, 0), (8, 8), (4, 8)])
 def test_compaction_wide(dev, kbytes, vbytes):
     import torch
