@@ -413,7 +413,8 @@ def _ms_allocate(num_elements: int, context: multisplit_context, kv: int) -> Non
            "multisplit_allocate")
     context.capacity = int(num_elements)
     context.size_sub_prob = int(lib().gms_subproblem_size(context.num_buckets, kv))
-    # no tile concept above 256 buckets (reduced-bit-sort path): 0 subproblems
+    # the reference's subproblem sizes stop at 256 buckets (subproblem_size_bms):
+    # 0 subproblems above
     context.num_sub_prob = context.num_blocks = (
         (int(num_elements) + context.size_sub_prob - 1) // context.size_sub_prob
         if context.size_sub_prob else 0)

@@ -293,7 +293,7 @@ __device__ __forceinline__ void reload_blocked(uint32_t (&key)[K], uint32_t (&va
 }
 
 // ---------------------------------------------------------------------------
-// Postscan: stable in-tile reorder + write-out, persistent with prefetch.
+// Postscan: stable in-tile reorder + write-out, one workgroup per tile.
 // ---------------------------------------------------------------------------
 // In-tile ranking of one tile: delta[b] = global start - local start of every
 // bucket run, then the stable LDS passes; leaves the tile in bucket order in
