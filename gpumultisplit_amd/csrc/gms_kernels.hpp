@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kThreads) void ms_prescan_kernel(const uint32_t* __
 // (thread t owns positions [t*K, t*K+K)); on exit s_keys/s_vals hold the tile
 // stably reordered by that digit.  Ends with a barrier.
 //
-// s_cnt is D x 256 u32, digit-major: word d*256 + t counts digit d among the
+// s_cnt is D x kThreads u32, digit-major: word d*kThreads + t counts digit d among the
 // keys of thread t.  Counting hits 64 distinct banks per wave; the scan reads
 // D consecutive words per thread, so position = exclusive prefix in memory
 // order = (keys of smaller digits) + (keys of this digit in earlier threads).
@@ -192,7 +192,7 @@ __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32
     constexpr uint32_t PAD = (uint32_t)cnt_pad(D), ROW = (uint32_t)cnt_row(D);
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
     const uint32_t first = tid * (uint32_t)K;
-    // LDS word of logical counter w = d*256 + t (rows of D padded to ROW).
+    // LDS word of logical counter w = d*kThreads + t (rows of D padded to ROW).
     auto cidx = [](uint32_t w) { return w + PAD * (w / (uint32_t)D); };
 
 #pragma unroll
@@ -205,7 +205,7 @@ __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32
         lr[k] = 0;
         if (FULL || first + (uint32_t)k < nvalid) {
             const uint32_t d = (f(key[k]) >> SHIFT) & DMASK;
-            lr[k] = atomicAdd(&s_cnt[cidx((d << 8) | tid)], 1u);
+            lr[k] = atomicAdd(&s_cnt[cidx(d * (uint32_t)kThreads + tid)], 1u);
             dg[k] = d;
         }
     }
@@ -252,7 +252,7 @@ __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32
     if constexpr (ALIAS) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const uint32_t pos = s_cnt[cidx((dg[k] << 8) | tid)] + lr[k];
+            const uint32_t pos = s_cnt[cidx(dg[k] * (uint32_t)kThreads + tid)] + lr[k];
             lr[k] = pos < nvalid ? pos : nvalid - 1u;  // in bounds even if counts were inconsistent
         }
         __syncthreads();
@@ -267,7 +267,7 @@ __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (FULL || first + (uint32_t)k < nvalid) {
-                uint32_t pos = s_cnt[cidx((dg[k] << 8) | tid)] + lr[k];
+                uint32_t pos = s_cnt[cidx(dg[k] * (uint32_t)kThreads + tid)] + lr[k];
                 pos = pos < nvalid ? pos : nvalid - 1u;  // in bounds even if counts were inconsistent
                 s_keys[pos] = key[k];
                 if constexpr (KV) s_vals[pos] = val[k];
