@@ -16,8 +16,9 @@ namespace gms {
 constexpr int kHistItems = 16;
 constexpr uint32_t kHistTile = kThreads * kHistItems;
 constexpr uint32_t kHistGrid = 512;
-constexpr uint32_t kRangeGrid = 2048;
+constexpr uint32_t kRangeGrid = 1024;
 constexpr int kHistCopies = 8;
+constexpr int kRangeCopies = 4;
 
 // bins_of(x, b) maps a register array of words to bins (>= nbins: not
 // counted); it sees a whole tile row at once so that a kernel can issue the
@@ -211,9 +212,11 @@ __global__ __launch_bounds__(kThreads) void hist_range_kernel(const uint32_t* __
     for (uint32_t i = 0; i < kCellsPerThread; i += 4)
         *reinterpret_cast<uint4*>(mine + i) = make_uint4(c[i], c[i + 1], c[i + 2], c[i + 3]);
     // (hist_body's first barrier orders the table before any lookup)
-    // One sub-histogram copy per wave: the cell table already takes most of
-    // a workgroup's share of LDS at this grid (8 workgroups per CU).
-    hist_body<VEC, 1, 1>(xbits, n, nb, d_hist, [&](auto& x, auto& b) {
+    // Four sub-histogram copies per wave (35 KB of LDS per workgroup with the
+    // cell table: the grid of kRangeGrid = 4 workgroups per CU is resident):
+    // sorted samples, whose lanes share a bin, contend on 4 addresses instead
+    // of one.
+    hist_body<VEC, 1, kRangeCopies>(xbits, n, nb, d_hist, [&](auto& x, auto& b) {
         constexpr int N = (int)(sizeof(x) / sizeof(x[0]));
         T v[N];
         uint32_t cnt[N];

@@ -164,6 +164,21 @@ def test_skewed_keys_histogram_and_multisplit(dev, kind):
             np.testing.assert_array_equal(offs, wo)
 
 
+def test_histogram_range_sorted_samples(dev):
+    import torch
+    # Sorted samples put whole waves into one bin (the kernel's histogram
+    # copies keep that from serialising); counts must stay exact.
+    g = W.rng(48)
+    n = (1 << 22) + 9
+    x = np.sort((g.random(n, dtype=np.float32) * np.float32(1024.0)).astype(np.float32))
+    lv = np.sort((g.random(257, dtype=np.float32) * np.float32(1024.0)).astype(np.float32))
+    lv[0], lv[-1] = 0.0, 1024.0
+    h = torch.zeros(256, dtype=torch.int32, device=dev)
+    gms.multisplit2_histogram_range(torch.from_numpy(x).to(dev), n, h, torch.from_numpy(lv).to(dev),
+                                    257)
+    np.testing.assert_array_equal(from_dev(h).view(np.int32), oracle.histogram_range(x, lv))
+
+
 def test_histogram_256_low_byte_accumulates(dev):
     import torch
     # 256-bin histogram of the low 8 bits, added twice into the same array
