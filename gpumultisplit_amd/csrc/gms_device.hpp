@@ -258,6 +258,25 @@ __device__ __forceinline__ void wave_count_add(uint32_t* cnt, uint32_t d, bool v
     if (pending) atomicAdd(&cnt[d], 1u);
 }
 
+// Whether the ids of the wave's lanes coincide: the id of the lowest valid
+// lane, or failing that of the next one, is shared by more than 8 lanes.  A
+// kernel probes this once per wave and tile and then counts with
+// wave_count_add (coinciding ids) or one LDS add per lane (spread ids), which
+// keeps the ballots off the path of evenly spread keys.
+__device__ __forceinline__ bool wave_ids_coincide(uint32_t d, bool valid) {
+    uint64_t rest = GMS_BALLOT(valid);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        if (rest == 0) break;  // the same in every lane
+        const int first = __builtin_ctzll(rest);
+        const uint32_t dl = (uint32_t)GMS_READLANE((int)d, first);
+        const uint64_t m = GMS_BALLOT(valid && d == dl);
+        if (__builtin_popcountll(m) > 8) return true;
+        rest &= ~m;
+    }
+    return false;
+}
+
 // Exclusive scan across a block of NW waves; `total` receives the block sum.
 // s_w must hold NW words; the call contains one __syncthreads and callers
 // must separate consecutive uses of the same s_w by a barrier.

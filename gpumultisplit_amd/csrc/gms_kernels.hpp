@@ -129,12 +129,18 @@ __device__ __forceinline__ void prescan_tile(const uint32_t (&x)[K], uint32_t nv
         uint32_t* my = s_h + wid * B;
         for (uint32_t i = lane; i < (uint32_t)B; i += 64) my[i] = 0;
         GMS_WAVE_SYNC();
+        // Probed on the wave's first key slot (key 4*tid of the tile).
+        const bool coincide = wave_ids_coincide(f(x[0]), FULL || 4u * tid < nvalid);
 #pragma unroll
         for (int j = 0; j < K / 4; ++j) {
             const uint32_t w4 = 4u * ((uint32_t)j * kThreads + tid);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                wave_count_add(my, f(x[4 * j + i]), FULL || w4 + i < nvalid);
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t d = f(x[4 * j + i]);
+                const bool v = FULL || w4 + i < nvalid;
+                if (coincide) wave_count_add(my, d, v);
+                else if (v) atomicAdd(&my[d], 1u);
+            }
         }
     }
     __syncthreads();

@@ -81,12 +81,18 @@ __global__ __launch_bounds__(kThreads) void ms_wide_prescan_kernel(const uint32_
     uint32_t* my = s_h + wid * B;
     for (uint32_t i = lane; i < (uint32_t)B; i += 64) my[i] = 0;
     GMS_WAVE_SYNC();
+    // Probed on the wave's first element slot (wave_ids_coincide).
+    const bool coincide = wave_ids_coincide(f(key_word_of<KW>(x, 0)), (4u * tid) / KW < nvalid);
 #pragma unroll
     for (int j = 0; j < WPT / 4; ++j) {
         const uint32_t e4 = (4u * ((uint32_t)j * kThreads + tid)) / KW;  // first element of the stripe
 #pragma unroll
-        for (int q = 0; q < 4 / KW; ++q)
-            wave_count_add(my, f(key_word_of<KW>(x + 4 * j, q)), e4 + (uint32_t)q < nvalid);
+        for (int q = 0; q < 4 / KW; ++q) {
+            const uint32_t d = f(key_word_of<KW>(x + 4 * j, q));
+            const bool v = e4 + (uint32_t)q < nvalid;
+            if (coincide) wave_count_add(my, d, v);
+            else if (v) atomicAdd(&my[d], 1u);
+        }
     }
     __syncthreads();
     for (uint32_t b = tid; b < (uint32_t)B; b += kThreads) {
