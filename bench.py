@@ -98,6 +98,21 @@ def bench_served(args, torch, gms, dev):
                           "p99_us": round(S.percentile(lat, 99), 1),
                           "p99_us_runs": [round(S.percentile(r[0], 99), 1) for r in runs]})
         out["poisson"] = rates
+        # Steady state: the request set replayed at each Poisson rate for at
+        # least 0.5 s of arrivals, over three copies of the output buffers.
+        sets = S.output_copies(gms, torch, recs, 3)
+        ladder = []
+        for j, frac in enumerate((0.3, 0.6, 0.8, 0.9, 0.95, 1.0, 1.1)):
+            r = S.run_steady(srv, sets, frac * cap, 0.5, seed=300 + j)
+            ladder.append({"offered_req_per_s": round(frac * cap, 1),
+                           "achieved_req_per_s": round(r["requests"] / r["wall_s"], 1),
+                           "gkeys_per_s": round(r["keys"] / r["wall_s"] / 1e9, 3),
+                           "p50_us": round(S.percentile(r["lat_us"], 50), 1),
+                           "p99_us": round(S.percentile(r["lat_us"], 99), 1),
+                           "requests": r["requests"], "arrivals_s": round(r["arrival_span_s"], 3),
+                           "wall_s": round(r["wall_s"], 3), "buffer_reuse_waits": r["reuse_waits"]})
+        del sets
+        out["steady"] = {"min_arrival_s": 0.5, "output_copies": 3, "ladder": ladder}
         st = srv.stats()
         out["steps"] = st["steps"]
         out["passes_per_step"] = round(st["step_items"] / max(st["steps"], 1), 2)
@@ -139,6 +154,12 @@ def bench_served(args, torch, gms, dev):
     out["plain_back_to_back"] = {"gkeys_per_s": round(total / wall_plain / 1e9, 3),
                                  "ms": round(wall_plain * 1e3, 3)}
     out["capacity_speedup_vs_plain"] = round(wall_plain / wall0, 3)
+    plain_gkeys = total / wall_plain / 1e9
+    ok = [x for x in out["steady"]["ladder"]
+          if x["achieved_req_per_s"] >= 0.98 * x["offered_req_per_s"]]
+    if ok:
+        best = max(ok, key=lambda x: x["offered_req_per_s"])
+        out["steady"]["max_sustained"] = dict(best, speedup_vs_plain=round(best["gkeys_per_s"] / plain_gkeys, 3))
     gms.multisplit_sort_release_memory(sort_ctx)
     for c in ms_ctx.values():
         gms.multisplit_release_memory(c)
@@ -357,14 +378,86 @@ def bench_aux(args, torch, gms, W, dev):
         step()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # GPU time of the calls alone: HIP events on the stream the library
+    # launches on, around a second pass of the same K calls
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    gpu_ms = e0.elapsed_time(e1) / args.steps
+    achieved = n * bpk / (gpu_ms * 1e-3) / 1e9
     out = {"metric": metric, "value": round(n / ms / 1e6, 3), "unit": "Gkeys/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 5),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "u64" if wide else "u32",
            "data": "synthetic (uniform keys)", "config": {"workload": w, "n": n},
            "call_hbm_fraction": round(n * bpk / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-           "algorithmic_bytes_per_key": bpk}
+           "algorithmic_bytes_per_key": bpk,
+           "roofline": {"kernel": "all kernels of one call", "bound": "hbm",
+                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "algorithmic_bytes": n * bpk, "avg_launch_us": round(gpu_ms * 1e3, 2)},
+           "cpu_baseline": None if args.no_cpu else cpu_baseline_aux(w, n, args.cpu_reps)}
     print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_aux(w, n, reps):
+    """The host-side baseline of an auxiliary row, single-threaded, median of
+    `reps` runs on uniform keys from the host generator (the same
+    distribution the GPU row uses): std::stable_sort for the sorts, a counting
+    loop for the histogram, std::stable_partition for compaction (the
+    reference drivers' CPU comparisons, oracle/baseline.cpp); the oracle's own
+    restatements for the custom-edge histogram and the 64-bit rows."""
+    import numpy as np
+    import oracle
+    from gpumultisplit_amd import workloads as W
+    sample = n
+    if w.startswith("sort"):
+        keys = W.uniform_keys(n, seed=1)
+        vals = np.arange(n, dtype=np.uint32) if w.endswith("kv") else None
+        run = lambda: oracle.baseline_stable_sort(keys, vals)
+        what = "std::stable_sort of (key, value) pairs" if vals is not None else "std::stable_sort of the keys"
+    elif w == "hist256":
+        keys = W.uniform_keys(n, seed=1)
+        run, what = (lambda: oracle.baseline_histogram(keys, 256, 0)), "counting loop over key & 0xFF"
+    elif w.startswith("compact"):
+        keys = W.uniform_keys(n, seed=1)
+        thr = int(int(w[7:]) / 100 * (1 << 32))
+        run, what = (lambda: oracle.baseline_stable_partition(keys, thr)), "std::stable_partition of the keys"
+    elif w == "histrange256":
+        sample = 1 << 24
+        g = W.rng(1)
+        x = (g.random(sample, dtype=np.float32) * np.float32(1024.0)).astype(np.float32)
+        lv = np.sort((g.random(257, dtype=np.float32) * np.float32(1024.0)).astype(np.float32))
+        lv[0], lv[-1] = 0.0, 1024.0
+        run, what = (lambda: oracle.histogram_range(x, lv)), "the oracle's edge scan per sample"
+    elif w in ("wide_kv_256", "wide_ko_256", "wide_hist256", "wide_sort8_kv"):
+        sample = 1 << 24 if w != "wide_sort8_kv" else 1 << 22
+        g = W.rng(1)
+        keys = g.integers(0, 1 << 63, sample, dtype=np.uint64) * np.uint64(2) + g.integers(0, 2, sample, dtype=np.uint64)
+        vals = np.arange(sample, dtype=np.uint64)
+        if w == "wide_hist256":
+            run = lambda: np.bincount(oracle.bucket_ids64(keys, 256, oracle.RANGE), minlength=256)
+            what = "the oracle's 64-bit range identifier + counting"
+        elif w == "wide_sort8_kv":
+            run, what = (lambda: oracle.sort_wide(keys, 8, vals)), "the oracle's LSD sort of 64-bit pairs"
+        else:
+            v = vals if w == "wide_kv_256" else None
+            run, what = (lambda: oracle.multisplit_wide(keys, 256, oracle.RANGE, 0, v)), \
+                "the oracle's 64-bit multisplit"
+    else:
+        return None
+    ts = []
+    for _ in range(reps):
+        c0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - c0)
+    med = statistics.median(ts)
+    return {"value": round(sample / med / 1e9, 4), "unit": "Gkeys/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} runs (median) of {what} over {sample} keys on the host",
+            "cpu_model": host_cpu_model()}
 
 
 def bench_variants(args, torch, gms, dev):
@@ -413,6 +506,17 @@ def bench_variants(args, torch, gms, dev):
                                   "wms_over_bms": round(res["wms"] / res["bms"], 3)}), flush=True)
         gms.set_variant(ctx, gms.VARIANT_AUTO)
         gms.multisplit_release_memory(ctx)
+
+
+def host_cpu_model():
+    """Model name of the host CPU (the cores the CPU baselines run on)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def traffic_from_profiles(workload):
@@ -501,9 +605,7 @@ def main():
         step()
     torch.cuda.synchronize()
 
-    stage_events = not args.no_stage_events
-    gms.set_profiling(ctx, stage_events)
-    gms.read_profile(ctx)  # reset
+    # The timed region: K calls with nothing else recorded on the stream.
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -511,9 +613,18 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
-    pre_ms, scan_ms, post_ms, calls = gms.read_profile(ctx) if stage_events else (0, 0, 0, 0)
-    gms.set_profiling(ctx, False)
     print(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step", file=sys.stderr, flush=True)
+    # Stage times for the roofline: a second pass of the same K calls with
+    # HIP events recorded on the library's stream around each stage.
+    stage_events = not args.no_stage_events
+    pre_ms = scan_ms = post_ms = calls = 0
+    if stage_events:
+        gms.set_profiling(ctx, True)
+        gms.read_profile(ctx)  # reset
+        for _ in range(args.steps):
+            step()
+        pre_ms, scan_ms, post_ms, calls = gms.read_profile(ctx)
+        gms.set_profiling(ctx, False)
 
     gkeys = n / (ms_per_step * 1e-3) / 1e9
     bpk_call = 20 if kv else 12
@@ -545,7 +656,8 @@ def main():
         med = statistics.median(ts)
         cpu = {"value": round(n / med / 1e9, 4), "unit": "Gkeys/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_reps} runs (median) of the full workload ({n} keys) with the "
-                         f"single-threaded C oracle on the host, {os.cpu_count()} host threads present"}
+                         f"single-threaded C oracle on the host, {os.cpu_count()} host threads present",
+               "cpu_model": host_cpu_model()}
 
     out = {
         "metric": f"{'key-value' if kv else 'key-only'} multisplit throughput, {n} uint32 keys, "

@@ -57,11 +57,25 @@ class gms_bucket_desc64(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_uint32), ("width", ctypes.c_uint32), ("param", ctypes.c_uint64)]
 
 
+class gms_stage_plan(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint32) for f in (
+        "num_buckets", "log_b", "key_words", "value_words", "n", "num_tiles", "large", "vec",
+        "done")] + [
+        ("d_matrix", ctypes.c_void_p), ("d_partials", ctypes.c_void_p),
+        ("d_staging", ctypes.c_void_p), ("scan_words", ctypes.c_uint64)]
+
+
 class gms_batch_item(ctypes.Structure):
     _fields_ = [("d_key_in", ctypes.c_void_p), ("d_value_in", ctypes.c_void_p),
                 ("d_key_out", ctypes.c_void_p), ("d_value_out", ctypes.c_void_p),
                 ("n", ctypes.c_uint32), ("num_buckets", ctypes.c_uint32),
                 ("desc", gms_bucket_desc), ("d_offsets", ctypes.c_void_p)]
+
+
+class gms_batch_sort_item(ctypes.Structure):
+    _fields_ = [("d_key_in", ctypes.c_void_p), ("d_value_in", ctypes.c_void_p),
+                ("d_key_out", ctypes.c_void_p), ("d_value_out", ctypes.c_void_p),
+                ("n", ctypes.c_uint32), ("digit_bits", ctypes.c_uint32)]
 
 
 class gms_batch_tuning(ctypes.Structure):
@@ -120,6 +134,10 @@ _SIGS = [
                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_U32)]),
     ("gms_multisplit_prescan_matrix", _I, [_P, _P, _U32, gms_bucket_desc, _I, _P, _P]),
     ("gms_exclusive_scan_u32", _I, [_P, _P, _P, _U32, _P]),
+    ("gms_multisplit_stage_plan", _I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, _P,
+                                       ctypes.POINTER(gms_stage_plan)]),
+    ("gms_stage_scan", _I, [ctypes.POINTER(gms_stage_plan), _P]),
+    ("gms_stage_offsets", _I, [ctypes.POINTER(gms_stage_plan), _P, _P]),
     ("gms_sort_create", _I, [_PP]),
     ("gms_sort_allocate", _I, [_P, _U32, _U32, _I]),
     ("gms_sort_release", _I, [_P]),
@@ -134,6 +152,8 @@ _SIGS = [
     ("gms_compaction", _I, [_P, _P, _P, _P, _P, _U32, gms_bucket_desc, _P, _P, _P]),
     ("gms_compaction_wide", _I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, gms_bucket_desc64, _P, _P,
                                  _P]),
+    ("gms_compaction_stage_plan", _I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, _P,
+                                       ctypes.POINTER(gms_stage_plan)]),
     ("gms_histogram_u32", _I, [_P, _U32, _P, _U32, gms_bucket_desc, _P]),
     ("gms_histogram_even_f32", _I, [_P, _U32, _P, _U32, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_float, _P]),
@@ -143,6 +163,8 @@ _SIGS = [
     ("gms_batch_create", _I, [_PP]),
     ("gms_batch_destroy", _I, [_P]),
     ("gms_batch_plan", _I, [_P, ctypes.POINTER(gms_batch_item), _U32, _P]),
+    ("gms_batch_plan_mixed", _I, [_P, ctypes.POINTER(gms_batch_item), _U32,
+                                  ctypes.POINTER(gms_batch_sort_item), _U32, _P]),
     ("gms_batch_run", _I, [_P, _P]),
     ("gms_batch_capture", _I, [_P]),
     ("gms_batch_launch_graph", _I, [_P, _P]),
@@ -196,6 +218,14 @@ def _check(status: int, what: str) -> None:
         raise GmsError(status, what)
 
 
+def _require_contiguous(t):
+    """The kernels read and write n consecutive elements from a buffer's
+    address: a strided view would give them the wrong elements, and an
+    expanded tensor (stride 0) has fewer elements of storage than it shows."""
+    if hasattr(t, "is_contiguous") and not t.is_contiguous():
+        raise ValueError("buffer must be contiguous (a strided or expanded view is not)")
+
+
 def _ptr(x, sizes=(4,)):
     """Device pointer of a torch tensor (or an int address); None stays None.
     Tensors must have elements of one of `sizes` bytes."""
@@ -204,6 +234,7 @@ def _ptr(x, sizes=(4,)):
     if isinstance(x, int):
         return x
     if hasattr(x, "data_ptr"):
+        _require_contiguous(x)
         if hasattr(x, "is_cuda") and not x.is_cuda:
             raise ValueError("device buffer expected, got a host tensor")
         if x.element_size() not in sizes:
@@ -448,6 +479,7 @@ def _offsets_buffer(multisplit_offset, num_buckets):
     if hasattr(multisplit_offset, "data_ptr") and not multisplit_offset.is_cuda:
         if multisplit_offset.numel() < num_buckets or multisplit_offset.element_size() != 4:
             raise ValueError("multisplit_offset needs num_buckets 32-bit entries")
+        _require_contiguous(multisplit_offset)
         return multisplit_offset.data_ptr(), None
     raise TypeError("multisplit_offset must be a host numpy array or CPU tensor")
 
@@ -775,27 +807,58 @@ class BatchItem:
                               _ptr(self.d_offsets))
 
 
+class BatchSortItem:
+    """One stable radix sort of a batch: the digit schedule of
+    ``multisplit_sort_<digit_bits>bit[_pairs]`` (src/api/multisplit_sort.cuh:
+    320-559), as :func:`sort`.  The outputs double as a pass buffer; the
+    input is never written."""
+
+    def __init__(self, d_key_in, d_key_out, num_elements: int, digit_bits: int = 8,
+                 d_value_in=None, d_value_out=None):
+        self.d_key_in, self.d_key_out = d_key_in, d_key_out
+        self.d_value_in, self.d_value_out = d_value_in, d_value_out
+        self.n = int(num_elements)
+        self.digit_bits = int(digit_bits)
+        for name in ("d_key_in", "d_key_out", "d_value_in", "d_value_out"):
+            _check_len(name, getattr(self, name), self.n)
+
+    def c_item(self) -> gms_batch_sort_item:
+        return gms_batch_sort_item(_ptr(self.d_key_in), _ptr(self.d_value_in),
+                                   _ptr(self.d_key_out), _ptr(self.d_value_out), self.n,
+                                   self.digit_bits)
+
+
 class Batch:
-    """A planned batch of independent multisplits.
+    """A planned batch of independent multisplits and radix sorts.
 
     ``run()`` launches it; ``capture()`` records the launches into a hipGraph
     that ``replay()`` relaunches; ``autotune()`` picks the tile shape and grid.
-    The items' tensors are kept alive by this object.
+    The items' tensors are kept alive by this object.  Sorts run as passes of
+    one batched multisplit per step: every multisplit and the first digit pass
+    of every sort in step 0, the p-th pass of every sort in step p.
     """
 
-    def __init__(self, items, stream=None):
+    def __init__(self, items, stream=None, sorts=()):
         h = ctypes.c_void_p()
         _check(lib().gms_batch_create(ctypes.byref(h)), "gms_batch_create")
         self._h = h
         self.items = list(items)
+        self.sorts = list(sorts)
         arr = (gms_batch_item * max(len(self.items), 1))(*[it.c_item() for it in self.items])
         self._stream = stream
-        _check(lib().gms_batch_plan(self._h, arr, len(self.items), self._s()), "gms_batch_plan")
+        if self.sorts:
+            sarr = (gms_batch_sort_item * len(self.sorts))(*[s.c_item() for s in self.sorts])
+            _check(lib().gms_batch_plan_mixed(self._h, arr, len(self.items), sarr,
+                                              len(self.sorts), self._s()),
+                   "gms_batch_plan_mixed")
+        else:
+            _check(lib().gms_batch_plan(self._h, arr, len(self.items), self._s()),
+                   "gms_batch_plan")
 
     def _s(self):
         if self._stream is not None:
             return self._stream
-        for it in self.items:
+        for it in self.items + self.sorts:
             s = _stream(it.d_key_in, it.d_key_out)
             if s is not None:
                 return s

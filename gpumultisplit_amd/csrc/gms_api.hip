@@ -209,16 +209,30 @@ static hipError_t with_functor(uint32_t B, gms_bucket_desc d, Fn&& fn) {
 
 static bool vec_aligned(const void* p) { return p == nullptr || (((uintptr_t)p) & 15u) == 0; }
 
+// The launch description of the plain stages for n keys in B buckets, or
+// GMS_ERR_CAPACITY when the scratch is too small for it.
+static gms_status plan_call(Scratch& s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                            uint32_t* vout, uint32_t n, uint32_t B, bool kv, hipStream_t stream,
+                            MsCall& c) {
+    const int lb = ceil_log2_u32(B);
+    const uint32_t tile = tile_keys(lb, kv, n);
+    const uint32_t nt = (uint32_t)(((size_t)n + tile - 1) / tile);
+    const size_t m = ((size_t)1 << lb) * nt;
+    if (m > s.hist_cap || scan_partials_needed(m) > s.part_cap) return GMS_ERR_CAPACITY;
+    c = MsCall{kin, vin, kout, vout, n, lb, kv, n >= kLongTileMin, nt, s.hist,
+               vec_aligned(kin) && vec_aligned(vin), stream};
+    return GMS_SUCCESS;
+}
+
 gms_status run_prescan_only(Scratch& s, const uint32_t* kin, uint32_t n, uint32_t B,
                             gms_bucket_desc d, bool kv, hipStream_t stream, uint32_t* nt_out) {
     const int lb = ceil_log2_u32(B);
     const uint32_t tile = tile_keys(lb, kv, n);
-    const uint32_t nt = (uint32_t)(((size_t)n + tile - 1) / tile);
-    if (nt_out) *nt_out = nt;
+    if (nt_out) *nt_out = (uint32_t)(((size_t)n + tile - 1) / tile);
     if (n == 0 || B <= 1) return GMS_SUCCESS;
-    if (((size_t)1 << lb) * nt > s.hist_cap) return GMS_ERR_CAPACITY;
-    MsCall c{kin, nullptr, nullptr, nullptr, n, lb, kv, n >= kLongTileMin, nt, s.hist,
-             vec_aligned(kin), stream};
+    MsCall c;
+    gms_status st = plan_call(s, kin, nullptr, nullptr, nullptr, n, B, kv, stream, c);
+    if (st != GMS_SUCCESS) return st;
     return hip_status(with_functor(B, d, [&](auto f) { return launch_prescan(c, f); }));
 }
 
@@ -236,24 +250,20 @@ static gms_status run_stages(Scratch& s, const uint32_t* kin, const uint32_t* vi
         if (e == hipSuccess && d_offsets) e = launch_fill(d_offsets, 0u, 1u, stream);
         return hip_status(e);
     }
-    const int lb = ceil_log2_u32(B);
-    const uint32_t tile = tile_keys(lb, kv, n);
-    const uint32_t nt = (uint32_t)(((size_t)n + tile - 1) / tile);
-    const size_t m = ((size_t)1 << lb) * nt;
-    if (m > s.hist_cap || scan_partials_needed(m) > s.part_cap) return GMS_ERR_CAPACITY;
-
-    MsCall c{kin, vin, kout, vout, n, lb, kv, n >= kLongTileMin, nt, s.hist,
-             vec_aligned(kin) && vec_aligned(vin), stream};
+    MsCall c;
+    gms_status st = plan_call(s, kin, vin, kout, vout, n, B, kv, stream, c);
+    if (st != GMS_SUCCESS) return st;
+    const uint32_t m = (uint32_t)(((size_t)1 << c.log_b) * c.num_tiles);
     hipEvent_t* ev = (prof && prof->on) ? prof->next(stream) : nullptr;
     hipError_t e = hipSuccess;
     if (ev) e = hipEventRecord(ev[0], stream);
     if (e == hipSuccess) e = with_f([&](auto f) { return launch_prescan(c, f); });
     if (e == hipSuccess && ev) e = hipEventRecord(ev[1], stream);
-    if (e == hipSuccess) e = launch_exclusive_scan(s.hist, s.hist, (uint32_t)m, s.part, stream);
+    if (e == hipSuccess) e = launch_exclusive_scan(s.hist, s.hist, m, s.part, stream);
     if (e == hipSuccess && ev) e = hipEventRecord(ev[2], stream);
     if (e == hipSuccess) e = with_f([&](auto f) { return launch_postscan(c, f); });
     if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
-    if (e == hipSuccess && d_offsets) e = launch_gather_offsets(s.hist, nt, B, d_offsets, stream);
+    if (e == hipSuccess && d_offsets) e = launch_gather_offsets(s.hist, c.num_tiles, B, d_offsets, stream);
     return hip_status(e);
 }
 
@@ -297,6 +307,76 @@ int sort_schedule(uint32_t digit_bits, SortPass* p) {
             return 0;
     }
     return k;
+}
+
+// ------------------------------------------------- stages of a caller's functor
+// The scratch layout of a multisplit whose identifier is compiled into the
+// caller's kernels (gpumultisplit_device.hpp); validation as gms_multisplit_wide.
+static gms_status stage_plan(Scratch& s, uint32_t capacity, int allocated, uint32_t B,
+                             uint32_t* staging, const void* kin, const void* vin, void* kout,
+                             void* vout, uint32_t n, uint32_t key_bytes, uint32_t value_bytes,
+                             hipStream_t stream, gms_stage_plan* plan) {
+    if (!plan) return GMS_ERR_INVALID_ARG;
+    *plan = gms_stage_plan{};
+    if ((key_bytes != 4 && key_bytes != 8) || (value_bytes != 0 && value_bytes != 4 && value_bytes != 8))
+        return GMS_ERR_INVALID_ARG;
+    if (!allocated) return GMS_ERR_NOT_ALLOCATED;
+    if (n > capacity) return GMS_ERR_CAPACITY;
+    const uint32_t kw = key_bytes / 4, vw = value_bytes / 4;
+    const bool wide = kw == 2 || vw == 2;
+    if (B > (wide ? 256u : kSinglePassBuckets)) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    if (value_bytes && n && (!vin || !vout)) return GMS_ERR_INVALID_ARG;
+    if (!value_bytes && (vin || vout)) return GMS_ERR_INVALID_ARG;
+    if (n) {
+        if (!kin || !kout) return GMS_ERR_INVALID_ARG;
+        if ((kw == 2 && (!aligned8(kin) || !aligned8(kout))) ||
+            (vw == 2 && (!aligned8(vin) || !aligned8(vout))))
+            return GMS_ERR_INVALID_ARG;
+        const size_t kb = (size_t)n * key_bytes, vb = (size_t)n * value_bytes;
+        if (ranges_overlap(kin, kb, kout, kb)) return GMS_ERR_ALIAS;
+        if (vw && (ranges_overlap(vin, vb, vout, vb) || ranges_overlap(kin, kb, vout, vb) ||
+                   ranges_overlap(vin, vb, kout, kb) || ranges_overlap(kout, kb, vout, vb)))
+            return GMS_ERR_ALIAS;
+    }
+    plan->num_buckets = B;
+    plan->key_words = kw;
+    plan->value_words = vw;
+    plan->n = n;
+    plan->d_partials = s.part;
+    plan->d_staging = staging;
+    if (n == 0 || B == 1) {  // nothing for the caller's kernels: one bucket is a copy
+        plan->done = 1;
+        hipError_t e = hipSuccess;
+        if (n) e = launch_copy(static_cast<const uint32_t*>(kin), static_cast<uint32_t*>(kout),
+                               (size_t)n * kw, stream);
+        if (e == hipSuccess && n && vw)
+            e = launch_copy(static_cast<const uint32_t*>(vin), static_cast<uint32_t*>(vout),
+                            (size_t)n * vw, stream);
+        return hip_status(e);
+    }
+    const int lb = ceil_log2_u32(B);
+    const size_t rows = (size_t)1 << lb;
+    plan->log_b = (uint32_t)lb;
+    plan->vec = vec_aligned(kin) && vec_aligned(vin);  // only the loads are vectors
+    plan->d_matrix = s.hist;
+    if (wide) {
+        const uint32_t tile = wide_tile_elems(kw, vw);
+        const uint32_t nt = (uint32_t)(((size_t)n + tile - 1) / tile);
+        if (rows * nt > s.hist_cap || scan_partials_needed(rows * nt) > s.part_cap)
+            return GMS_ERR_CAPACITY;
+        plan->num_tiles = nt;
+        plan->scan_words = rows * nt;
+        return GMS_SUCCESS;
+    }
+    MsCall c;
+    gms_status st = plan_call(s, static_cast<const uint32_t*>(kin), static_cast<const uint32_t*>(vin),
+                              static_cast<uint32_t*>(kout), static_cast<uint32_t*>(vout), n, B,
+                              vw != 0, stream, c);
+    if (st != GMS_SUCCESS) return st;
+    plan->num_tiles = c.num_tiles;
+    plan->large = c.large ? 1u : 0u;
+    plan->scan_words = rows * c.num_tiles;
+    return GMS_SUCCESS;
 }
 
 }  // namespace gms
@@ -369,9 +449,18 @@ gms_status gms_multisplit_create(uint32_t num_buckets, gms_multisplit_ctx** out)
     return GMS_SUCCESS;
 }
 
+static void free_wide_temporaries(gms_multisplit_ctx* ctx) {
+    if (ctx->wide_k) (void)hipFree(ctx->wide_k);
+    if (ctx->wide_v) (void)hipFree(ctx->wide_v);
+    ctx->wide_k = ctx->wide_v = nullptr;
+}
+
 gms_status gms_multisplit_allocate(gms_multisplit_ctx* ctx, uint32_t n_capacity, int key_value) {
     (void)key_value;  // the matrix is sized for both tile shapes
     if (!ctx) return GMS_ERR_INVALID_ARG;
+    // The 64-bit pass temporaries hold capacity x 2 words: a new capacity
+    // drops them, and the next call that needs them allocates them anew.
+    free_wide_temporaries(ctx);
     const bool large = ctx->num_buckets > 256;
     gms_status st = scratch_alloc(ctx->s, matrix_elems(n_capacity, ctx->num_buckets));
     if (st == GMS_SUCCESS && large)
@@ -388,9 +477,7 @@ gms_status gms_multisplit_allocate(gms_multisplit_ctx* ctx, uint32_t n_capacity,
 
 gms_status gms_multisplit_release(gms_multisplit_ctx* ctx) {
     if (!ctx) return GMS_ERR_INVALID_ARG;
-    if (ctx->wide_k) (void)hipFree(ctx->wide_k);
-    if (ctx->wide_v) (void)hipFree(ctx->wide_v);
-    ctx->wide_k = ctx->wide_v = nullptr;
+    free_wide_temporaries(ctx);
     scratch_free(ctx->s);
     large_scratch_free(ctx->large);
     ctx->prof.destroy();
@@ -630,6 +717,34 @@ gms_status gms_exclusive_scan_u32(gms_multisplit_ctx* ctx, const uint32_t* d_in,
     if (d_in != d_out && ranges_overlap(d_in, (size_t)n * 4, d_out, (size_t)n * 4))
         return GMS_ERR_ALIAS;
     return hip_status(launch_exclusive_scan(d_in, d_out, n, ctx->s.part, (hipStream_t)stream));
+}
+
+gms_status gms_multisplit_stage_plan(gms_multisplit_ctx* ctx, const void* kin, const void* vin,
+                                     void* kout, void* vout, uint32_t n, uint32_t key_bytes,
+                                     uint32_t value_bytes, void* stream, gms_stage_plan* plan) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    uint32_t* staging = ctx->num_buckets > 256 ? ctx->large.offs : ctx->s.offs;
+    return stage_plan(ctx->s, ctx->capacity, ctx->allocated, ctx->num_buckets, staging, kin, vin,
+                      kout, vout, n, key_bytes, value_bytes, (hipStream_t)stream, plan);
+}
+
+gms_status gms_stage_scan(const gms_stage_plan* plan, void* stream) {
+    if (!plan) return GMS_ERR_INVALID_ARG;
+    if (plan->done) return GMS_SUCCESS;
+    if (!plan->d_matrix || !plan->d_partials || plan->scan_words > 0xFFFFFFFFull)
+        return GMS_ERR_INVALID_ARG;
+    return hip_status(launch_exclusive_scan(plan->d_matrix, plan->d_matrix,
+                                            (uint32_t)plan->scan_words, plan->d_partials,
+                                            (hipStream_t)stream));
+}
+
+gms_status gms_stage_offsets(const gms_stage_plan* plan, uint32_t* d_out, void* stream) {
+    if (!plan || !d_out) return GMS_ERR_INVALID_ARG;
+    hipStream_t s = (hipStream_t)stream;
+    if (plan->done) return hip_status(launch_fill(d_out, 0u, plan->num_buckets, s));
+    // bucket b starts at the first word of its row of the scanned matrix
+    return hip_status(launch_gather_offsets(plan->d_matrix, plan->num_tiles, plan->num_buckets,
+                                            d_out, s));
 }
 
 // ---------------------------------------------------------------------- sort
@@ -902,6 +1017,14 @@ gms_status gms_compaction_wide(gms_compaction_ctx* ctx, const void* kin, const v
         return hip_status(hipStreamSynchronize(s));
     }
     return GMS_SUCCESS;
+}
+
+gms_status gms_compaction_stage_plan(gms_compaction_ctx* ctx, const void* kin, const void* vin,
+                                     void* kout, void* vout, uint32_t n, uint32_t key_bytes,
+                                     uint32_t value_bytes, void* stream, gms_stage_plan* plan) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    return stage_plan(ctx->s, ctx->capacity, ctx->allocated, 2, ctx->s.offs, kin, vin, kout, vout,
+                      n, key_bytes, value_bytes, (hipStream_t)stream, plan);
 }
 
 // ----------------------------------------------------------------- histogram

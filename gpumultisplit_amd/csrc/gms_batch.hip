@@ -2,6 +2,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <new>
 
 #include "gms_batch.hpp"
@@ -291,11 +292,17 @@ void BatchEngine::release() {
 
 using namespace gms;
 
+// A batch is a sequence of steps, each one batched multisplit (BatchEngine):
+// step 0 holds the multisplits and the first pass of every sort, step p the
+// p-th pass of the sorts that have one.
 struct gms_batch {
-    BatchEngine eng;
-    DevVec<uint8_t> mem;
-    BatchDevice dev;
+    std::vector<std::unique_ptr<BatchEngine>> eng;  // one per step
+    std::vector<BatchDevice> dev;
+    DevVec<uint8_t> mem;  // every step's tables
+    DevVec<uint8_t> tmp;  // the sorts' second pass buffers
     std::vector<gms_batch_item> items;
+    std::vector<gms_batch_sort_item> sorts;
+    BatchTuning tuning;
     bool planned = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -307,18 +314,86 @@ struct gms_batch {
         exec = nullptr;
         graph = nullptr;
     }
+    void release() {
+        for (auto& e : eng) e->release();
+        eng.clear();
+        dev.clear();
+        mem.release();
+        tmp.release();
+    }
+    // The items of every step.  Pass j of a sort with np passes reads the
+    // previous pass's output (the input for j = 0) and writes the output
+    // buffers when np-1-j is even, the batch's pass buffer otherwise, so that
+    // the last pass writes the output and the input is never written
+    // (multisplit_sort.cuh:320-559).
+    gms_status step_items(std::vector<std::vector<gms_batch_item>>& st, hipStream_t s) {
+        size_t words = 0;
+        for (const auto& q : sorts) words += ((size_t)q.n + 63) / 64 * 64 * (q.d_value_in ? 2 : 1);
+        gms_status r = tmp.reserve(std::max<size_t>(words, 1) * sizeof(uint32_t), s);
+        if (r != GMS_SUCCESS) return r;
+        st.clear();
+        if (!items.empty()) st.emplace_back(items);
+        uint32_t* base = reinterpret_cast<uint32_t*>(tmp.p);
+        for (const auto& q : sorts) {
+            SortPass pass[8];
+            const int np = sort_schedule(q.digit_bits, pass);
+            if (q.n == 0) continue;
+            const size_t w = ((size_t)q.n + 63) / 64 * 64;
+            uint32_t* tk = base;
+            uint32_t* tv = q.d_value_in ? base + w : nullptr;
+            base += w * (q.d_value_in ? 2 : 1);
+            const uint32_t* sk = q.d_key_in;
+            const uint32_t* sv = q.d_value_in;
+            for (int j = 0; j < np; ++j) {
+                const bool to_out = ((np - 1 - j) % 2) == 0;
+                uint32_t* dk = to_out ? q.d_key_out : tk;
+                uint32_t* dv = q.d_value_in ? (to_out ? q.d_value_out : tv) : nullptr;
+                if ((int)st.size() <= j) st.emplace_back();
+                st[j].push_back(gms_batch_item{sk, sv, dk, dv, q.n, 1u << pass[j].bits,
+                                               gms_bucket_desc{GMS_BUCKET_DIGIT, pass[j].shift,
+                                                               pass[j].bits},
+                                               nullptr});
+                sk = dk;
+                sv = dv;
+            }
+        }
+        return GMS_SUCCESS;
+    }
     gms_status replan(hipStream_t s) {
         drop_graph();
         planned = false;
-        gms_status st = eng.plan(items.data(), (uint32_t)items.size(), 0);
-        if (st == GMS_SUCCESS) st = mem.reserve(eng.device_bytes(), s);
-        if (st == GMS_SUCCESS) {
-            dev = eng.carve(mem.p);
-            st = eng.upload(dev, s);
+        std::vector<std::vector<gms_batch_item>> st;
+        gms_status r = step_items(st, s);
+        if (r != GMS_SUCCESS) return r;
+        while (eng.size() < st.size()) eng.push_back(std::make_unique<BatchEngine>());
+        while (eng.size() > st.size()) {
+            eng.back()->release();
+            eng.pop_back();
         }
-        if (st == GMS_SUCCESS) st = hip_status(hipStreamSynchronize(s));
-        planned = st == GMS_SUCCESS;
-        return st;
+        size_t bytes = 0;
+        for (size_t i = 0; i < st.size() && r == GMS_SUCCESS; ++i) {
+            eng[i]->tuning = tuning;
+            r = eng[i]->plan(st[i].data(), (uint32_t)st[i].size(), 0);
+            bytes += (eng[i]->device_bytes() + 255) & ~(size_t)255;
+        }
+        if (r == GMS_SUCCESS) r = mem.reserve(std::max<size_t>(bytes, 256), s);
+        dev.assign(st.size(), BatchDevice{});
+        char* p = reinterpret_cast<char*>(mem.p);
+        for (size_t i = 0; i < st.size() && r == GMS_SUCCESS; ++i) {
+            dev[i] = eng[i]->carve(p);
+            p += (eng[i]->device_bytes() + 255) & ~(size_t)255;
+            r = eng[i]->upload(dev[i], s);
+        }
+        if (r == GMS_SUCCESS) r = hip_status(hipStreamSynchronize(s));
+        planned = r == GMS_SUCCESS;
+        return r;
+    }
+    gms_status launch(hipStream_t s) {
+        for (size_t i = 0; i < eng.size(); ++i) {
+            const gms_status r = eng[i]->launch(dev[i], s);
+            if (r != GMS_SUCCESS) return r;
+        }
+        return GMS_SUCCESS;
     }
 };
 
@@ -334,29 +409,46 @@ gms_status gms_batch_destroy(gms_batch* b) {
     if (!b) return GMS_SUCCESS;
     b->drop_graph();
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
-    b->eng.release();
-    b->mem.release();
+    b->release();
     delete b;
     return GMS_SUCCESS;
 }
 
-gms_status gms_batch_plan(gms_batch* b, const gms_batch_item* items, uint32_t count,
-                          void* stream) {
-    if (!b || (count && !items)) return GMS_ERR_INVALID_ARG;
+gms_status gms_batch_plan_mixed(gms_batch* b, const gms_batch_item* items, uint32_t count,
+                                const gms_batch_sort_item* sorts, uint32_t num_sorts,
+                                void* stream) {
+    if (!b || (count && !items) || (num_sorts && !sorts)) return GMS_ERR_INVALID_ARG;
     for (uint32_t i = 0; i < count; ++i) {
         gms_status st = check_batch_item(items[i]);
         if (st != GMS_SUCCESS) return st;
     }
-    gms_status st = check_batch_overlaps(items, count);
+    // every range of the batch, sorts as items without offsets
+    std::vector<gms_batch_item> all(items, items + count);
+    for (uint32_t i = 0; i < num_sorts; ++i) {
+        const gms_batch_sort_item& q = sorts[i];
+        SortPass pass[8];
+        if (sort_schedule(q.digit_bits, pass) == 0) return GMS_ERR_INVALID_ARG;
+        if ((q.d_value_in == nullptr) != (q.d_value_out == nullptr)) return GMS_ERR_INVALID_ARG;
+        if (q.n && (!q.d_key_in || !q.d_key_out)) return GMS_ERR_INVALID_ARG;
+        all.push_back(gms_batch_item{q.d_key_in, q.d_value_in, q.d_key_out, q.d_value_out, q.n, 1,
+                                     gms_bucket_desc{GMS_BUCKET_RANGE, 0, 0}, nullptr});
+    }
+    gms_status st = check_batch_overlaps(all.data(), (uint32_t)all.size());
     if (st != GMS_SUCCESS) return st;
     b->items.assign(items, items + count);
+    b->sorts.assign(sorts, sorts + num_sorts);
     return b->replan((hipStream_t)stream);
+}
+
+gms_status gms_batch_plan(gms_batch* b, const gms_batch_item* items, uint32_t count,
+                          void* stream) {
+    return gms_batch_plan_mixed(b, items, count, nullptr, 0, stream);
 }
 
 gms_status gms_batch_run(gms_batch* b, void* stream) {
     if (!b) return GMS_ERR_INVALID_ARG;
     if (!b->planned) return GMS_ERR_NOT_ALLOCATED;
-    return b->eng.launch(b->dev, (hipStream_t)stream);
+    return b->launch((hipStream_t)stream);
 }
 
 gms_status gms_batch_capture(gms_batch* b) {
@@ -369,7 +461,7 @@ gms_status gms_batch_capture(gms_batch* b) {
     }
     if (hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return GMS_ERR_HIP;
-    gms_status st = b->eng.launch(b->dev, b->cap_stream);
+    gms_status st = b->launch(b->cap_stream);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
     if (st != GMS_SUCCESS || e != hipSuccess) {
@@ -397,15 +489,15 @@ gms_status gms_batch_set_tuning(gms_batch* b, gms_batch_tuning t) {
         t.items_per_thread != 32)
         return GMS_ERR_INVALID_ARG;
     if (t.persistent > 1) return GMS_ERR_INVALID_ARG;
-    b->eng.tuning.items_per_thread = t.items_per_thread;
-    b->eng.tuning.persistent = t.persistent;
+    b->tuning.items_per_thread = t.items_per_thread;
+    b->tuning.persistent = t.persistent;
     return b->planned ? b->replan(nullptr) : GMS_SUCCESS;
 }
 
 gms_status gms_batch_get_tuning(const gms_batch* b, gms_batch_tuning* t) {
     if (!b || !t) return GMS_ERR_INVALID_ARG;
-    t->items_per_thread = b->eng.tuning.items_per_thread;
-    t->persistent = b->eng.tuning.persistent;
+    t->items_per_thread = b->tuning.items_per_thread;
+    t->persistent = b->tuning.persistent;
     return GMS_SUCCESS;
 }
 
@@ -419,18 +511,18 @@ gms_status gms_batch_autotune(gms_batch* b, uint32_t reps, void* stream, gms_bat
         if (e0) (void)hipEventDestroy(e0);
         return GMS_ERR_HIP;
     }
-    const BatchTuning keep = b->eng.tuning;
+    const BatchTuning keep = b->tuning;
     BatchTuning best = keep;
     float best_ms = -1.f;
     gms_status st = GMS_SUCCESS;
     for (uint32_t K : {8u, 16u, 24u, 32u}) {
         for (uint32_t pers : {0u, 1u}) {
-            b->eng.tuning.items_per_thread = K;
-            b->eng.tuning.persistent = pers;
+            b->tuning.items_per_thread = K;
+            b->tuning.persistent = pers;
             st = b->replan(s);
-            if (st == GMS_SUCCESS) st = b->eng.launch(b->dev, s);  // warm-up
+            if (st == GMS_SUCCESS) st = b->launch(s);  // warm-up
             if (st == GMS_SUCCESS) st = hip_status(hipEventRecord(e0, s));
-            for (uint32_t r = 0; r < reps && st == GMS_SUCCESS; ++r) st = b->eng.launch(b->dev, s);
+            for (uint32_t r = 0; r < reps && st == GMS_SUCCESS; ++r) st = b->launch(s);
             if (st == GMS_SUCCESS) st = hip_status(hipEventRecord(e1, s));
             if (st == GMS_SUCCESS) st = hip_status(hipEventSynchronize(e1));
             float ms = 0.f;
@@ -439,14 +531,14 @@ gms_status gms_batch_autotune(gms_batch* b, uint32_t reps, void* stream, gms_bat
             ms /= (float)reps;
             if (best_ms < 0.f || ms < best_ms) {
                 best_ms = ms;
-                best = b->eng.tuning;
+                best = b->tuning;
             }
         }
         if (st != GMS_SUCCESS) break;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    b->eng.tuning = st == GMS_SUCCESS ? best : keep;
+    b->tuning = st == GMS_SUCCESS ? best : keep;
     const gms_status st2 = b->replan(s);
     if (st != GMS_SUCCESS) return st;
     if (st2 != GMS_SUCCESS) return st2;
@@ -461,10 +553,18 @@ gms_status gms_batch_autotune(gms_batch* b, uint32_t reps, void* stream, gms_bat
 gms_status gms_batch_stats(const gms_batch* b, uint64_t* keys, uint32_t* work_items,
                            uint64_t* matrix_words, uint32_t* launches) {
     if (!b) return GMS_ERR_INVALID_ARG;
-    if (keys) *keys = b->eng.keys_total;
-    if (work_items) *work_items = b->eng.num_work;
-    if (matrix_words) *matrix_words = b->eng.m_total;
-    if (launches) *launches = b->eng.launches();
+    uint64_t k = 0, m = 0;
+    uint32_t w = 0, l = 0;
+    for (const auto& e : b->eng) {
+        k += e->keys_total;
+        m += e->m_total;
+        w += e->num_work;
+        l += e->launches();
+    }
+    if (keys) *keys = k;
+    if (work_items) *work_items = w;
+    if (matrix_words) *matrix_words = m;
+    if (launches) *launches = l;
     return GMS_SUCCESS;
 }
 

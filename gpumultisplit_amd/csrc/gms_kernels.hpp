@@ -102,11 +102,12 @@ __device__ __forceinline__ void load_striped4_guarded(const uint32_t* __restrict
 // ---------------------------------------------------------------------------
 // Prescan: per-tile bucket counts.
 // ---------------------------------------------------------------------------
-// Counts of one tile into s_h (per-wave sub-histograms), then H[:, t].
+// Counts of one tile into s_h, kWaves rows of B words: row q receives wave
+// q's counts.  Up to 8 buckets the rows are overwritten; above that the
+// waves add into them with LDS atomics, so the rows must be zero on entry.
 template <int LOG_B, int K, bool FULL, class F>
-__device__ __forceinline__ void prescan_tile(const uint32_t (&x)[K], uint32_t nvalid, uint32_t t,
-                                             uint32_t num_tiles, uint32_t* __restrict__ H, F f,
-                                             uint32_t* s_h) {
+__device__ __forceinline__ void prescan_count(const uint32_t (&x)[K], uint32_t nvalid, F f,
+                                              uint32_t* s_h) {
     constexpr int B = 1 << LOG_B;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
     if constexpr (LOG_B <= 3) {
@@ -125,10 +126,7 @@ __device__ __forceinline__ void prescan_tile(const uint32_t (&x)[K], uint32_t nv
             if (lane == 63u) s_h[wid * B + b] = c;
         }
     } else {
-        // Per-wave sub-histograms; a wave zeroes and fills only its own.
         uint32_t* my = s_h + wid * B;
-        for (uint32_t i = lane; i < (uint32_t)B; i += 64) my[i] = 0;
-        GMS_WAVE_SYNC();
         // Probed on the wave's first key slot (key 4*tid of the tile).
         const bool coincide = wave_ids_coincide(f(x[0]), FULL || 4u * tid < nvalid);
 #pragma unroll
@@ -143,6 +141,23 @@ __device__ __forceinline__ void prescan_tile(const uint32_t (&x)[K], uint32_t nv
             }
         }
     }
+}
+
+// Counts of one tile into H[:, t] (one column of the bucket-major per-tile
+// count matrix; the batched kernels' layout).
+template <int LOG_B, int K, bool FULL, class F>
+__device__ __forceinline__ void prescan_tile(const uint32_t (&x)[K], uint32_t nvalid, uint32_t t,
+                                             uint32_t num_tiles, uint32_t* __restrict__ H, F f,
+                                             uint32_t* s_h) {
+    constexpr int B = 1 << LOG_B;
+    const uint32_t tid = threadIdx.x;
+    if constexpr (LOG_B > 3) {
+        // Per-wave sub-histograms; a wave zeroes and fills only its own.
+        uint32_t* my = s_h + (tid >> 6) * B;
+        for (uint32_t i = tid & 63u; i < (uint32_t)B; i += 64) my[i] = 0;
+        GMS_WAVE_SYNC();
+    }
+    prescan_count<LOG_B, K, FULL>(x, nvalid, f, s_h);
     __syncthreads();
     for (uint32_t b = tid; b < (uint32_t)B; b += kThreads) {
         uint32_t s = 0;
@@ -153,6 +168,9 @@ __device__ __forceinline__ void prescan_tile(const uint32_t (&x)[K], uint32_t nv
     __syncthreads();
 }
 
+// Prescan, one workgroup per tile: H[b * T + t] = count of bucket b in tile
+// t.  Tiles are handed to workgroups in the XCD order (xcd_tile), so that
+// neighbouring tiles' columns of H share L2 lines.
 template <int LOG_B, int K, bool VEC, class F>
 __global__ __launch_bounds__(kThreads) void ms_prescan_kernel(const uint32_t* __restrict__ kin,
                                                               uint32_t n, uint32_t num_tiles,
@@ -160,7 +178,6 @@ __global__ __launch_bounds__(kThreads) void ms_prescan_kernel(const uint32_t* __
     constexpr int B = 1 << LOG_B;
     constexpr uint32_t TILE = kThreads * K;
     __shared__ __attribute__((aligned(16))) uint32_t s_h[kWaves * B];
-
     const uint32_t t = xcd_tile(blockIdx.x, num_tiles);
     const uint32_t nvalid = tile_valid(n, t, TILE);
     uint32_t x[K];
@@ -204,15 +221,17 @@ __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32
 #pragma unroll
     for (int d = 0; d < D; ++d) s_cnt[cidx((uint32_t)d * kThreads + tid)] = 0;
 
-    uint32_t lr[K], dg[K];
+    // slot[k] = (LDS word of key k's counter) << 8 | (its rank among this
+    // thread's keys of the same digit, < K <= 255): one register per key.
+    static_assert(K <= 255, "ranks among a thread's keys fit in 8 bits");
+    uint32_t slot[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        dg[k] = 0;
-        lr[k] = 0;
+        slot[k] = cidx(tid) << 8;
         if (FULL || first + (uint32_t)k < nvalid) {
             const uint32_t d = (f(key[k]) >> SHIFT) & DMASK;
-            lr[k] = atomicAdd(&s_cnt[cidx(d * (uint32_t)kThreads + tid)], 1u);
-            dg[k] = d;
+            const uint32_t w = cidx(d * (uint32_t)kThreads + tid);
+            slot[k] = (w << 8) | atomicAdd(&s_cnt[w], 1u);
         }
     }
     __syncthreads();
@@ -258,22 +277,22 @@ __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32
     if constexpr (ALIAS) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const uint32_t pos = s_cnt[cidx(dg[k] * (uint32_t)kThreads + tid)] + lr[k];
-            lr[k] = pos < nvalid ? pos : nvalid - 1u;  // in bounds even if counts were inconsistent
+            const uint32_t pos = s_cnt[slot[k] >> 8] + (slot[k] & 0xFFu);
+            slot[k] = pos < nvalid ? pos : nvalid - 1u;  // in bounds even if counts were inconsistent
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (FULL || first + (uint32_t)k < nvalid) {
-                s_keys[lr[k]] = key[k];
-                if constexpr (KV) s_vals[lr[k]] = val[k];
+                s_keys[slot[k]] = key[k];
+                if constexpr (KV) s_vals[slot[k]] = val[k];
             }
         }
     } else {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             if (FULL || first + (uint32_t)k < nvalid) {
-                uint32_t pos = s_cnt[cidx(dg[k] * (uint32_t)kThreads + tid)] + lr[k];
+                uint32_t pos = s_cnt[slot[k] >> 8] + (slot[k] & 0xFFu);
                 pos = pos < nvalid ? pos : nvalid - 1u;  // in bounds even if counts were inconsistent
                 s_keys[pos] = key[k];
                 if constexpr (KV) s_vals[pos] = val[k];
@@ -358,7 +377,10 @@ __device__ __forceinline__ void postscan_write(uint32_t nvalid, uint32_t n,
                                                const uint32_t* s_keys, const uint32_t* s_vals,
                                                const uint32_t* s_delta) {
     const uint32_t tid = threadIdx.x;
-#pragma unroll
+    // Unrolled by 4, not fully: the compiler would otherwise compute every
+    // store's 64-bit address up front, which costs the registers a caller
+    // keeps its next tile in.
+#pragma unroll 4
     for (int j = 0; j < K; ++j) {
         const uint32_t p = (uint32_t)j * kThreads + tid;
         if (FULL || p < nvalid) {
@@ -371,6 +393,18 @@ __device__ __forceinline__ void postscan_write(uint32_t nvalid, uint32_t n,
     }
 }
 
+// Word `off` of the count matrix.  The matrix of one call is below 4 GB (at
+// most 1024 rows of 2^32 / 6144 tiles), so the byte offset is 32-bit: one
+// register per address instead of two.
+__device__ __forceinline__ uint32_t matrix_word(const uint32_t* __restrict__ m, uint32_t off) {
+    return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(m) + (off << 2));
+}
+
+// Postscan, one workgroup per tile, tiles in the XCD order (xcd_tile).  The
+// same launch shape as the prescan: on MI355X a workgroup per tile, handed
+// out by the dispatcher in tile order, measured faster than resident
+// workgroups that walk several tiles each with the next one prefetched
+// (426 against 800 us at 2^27 key-value pairs, 256 buckets).
 template <int LOG_B, int K, int LEVEL_BITS, bool KV, bool VEC, class F>
 __global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
@@ -405,16 +439,16 @@ __global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
     // buckets tid*R .. tid*R+R-1 (threads past B reload bucket B-1 and ignore
     // it).
     constexpr int R = (B + kThreads - 1) / kThreads;
-    const size_t mtot = (size_t)B * num_tiles;
+    const uint32_t mtot = (uint32_t)B * num_tiles;
     uint32_t g0[R], g1raw[R];
     bool last[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const uint32_t b = tid * (uint32_t)R + (uint32_t)r;
-        const size_t idx = (size_t)(b < (uint32_t)B ? b : (uint32_t)B - 1u) * num_tiles + t;
+        const uint32_t idx = (b < (uint32_t)B ? b : (uint32_t)B - 1u) * num_tiles + t;
         last[r] = idx + 1 >= mtot;
-        g0[r] = G[idx];
-        g1raw[r] = G[last[r] ? idx : idx + 1];
+        g0[r] = matrix_word(G, idx);
+        g1raw[r] = matrix_word(G, last[r] ? idx : idx + 1);
     }
     auto tile_counts = [&](uint32_t (&cnt)[R]) {
 #pragma unroll
@@ -435,7 +469,7 @@ __global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
         uint32_t cnt[R];
         tile_counts(cnt);
         postscan_rank<LOG_B, K, LEVEL_BITS, KV, true, ALIAS>(key, val, g0, cnt, TILE, f, s_keys,
-                                                      s_vals, s_cnt, s_delta, s_bs, s_ws);
+                                                             s_vals, s_cnt, s_delta, s_bs, s_ws);
         postscan_write<K, KV, true>(TILE, n, kout, vout, f, s_keys, s_vals, s_delta);
     } else {
         load_blocked_guarded<K>(kin, base, nvalid, key);
@@ -447,9 +481,45 @@ __global__ __launch_bounds__(kThreads) void ms_postscan_kernel(
         uint32_t cnt[R];
         tile_counts(cnt);
         postscan_rank<LOG_B, K, LEVEL_BITS, KV, false, ALIAS>(key, val, g0, cnt, nvalid, f, s_keys,
-                                                       s_vals, s_cnt, s_delta, s_bs, s_ws);
+                                                              s_vals, s_cnt, s_delta, s_bs, s_ws);
         postscan_write<K, KV, false>(nvalid, n, kout, vout, f, s_keys, s_vals, s_delta);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Bucket starts of a finished multisplit output of n elements of KW words:
+// start[b] = the first i with f(element i) >= b, which is n when no element
+// is.  The ids along the output are non-decreasing, so one thread per bucket
+// binary-searches for it: at most ceil(log2(n + 1)) steps whatever the
+// distribution (no thread walks a run of empty buckets).  Neighbouring
+// buckets follow the same first steps, which share cache lines.
+// ---------------------------------------------------------------------------
+template <int KW, class F>
+__global__ __launch_bounds__(kThreads) void bucket_starts_kernel(const uint32_t* __restrict__ out,
+                                                                 uint32_t n, uint32_t num_buckets,
+                                                                 F f, uint32_t* __restrict__ start) {
+    const uint32_t b = blockIdx.x * kThreads + threadIdx.x;
+    if (b >= num_buckets) return;
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        uint32_t id;
+        if constexpr (KW == 2)
+            id = f((uint64_t)out[2 * (size_t)mid] | ((uint64_t)out[2 * (size_t)mid + 1] << 32));
+        else
+            id = f(out[mid]);
+        if (id < b) lo = mid + 1u;
+        else hi = mid;
+    }
+    start[b] = lo;
+}
+
+template <int KW, class F>
+hipError_t launch_bucket_starts(const uint32_t* out, uint32_t n, uint32_t num_buckets, const F& f,
+                                uint32_t* start, hipStream_t stream) {
+    hipLaunchKernelGGL((bucket_starts_kernel<KW, F>), dim3((num_buckets + kThreads - 1) / kThreads),
+                       dim3(kThreads), 0, stream, out, n, num_buckets, f, start);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

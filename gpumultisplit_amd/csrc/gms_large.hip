@@ -15,23 +15,6 @@
 
 namespace gms {
 
-// Thread i writes the starts of the buckets in (f(kout[i-1]), f(kout[i])];
-// the thread of the last element also writes those above f(kout[n-1]).
-// f(kout[.]) is non-decreasing and below num_buckets.
-__global__ __launch_bounds__(kThreads) void bucket_starts_kernel(const uint32_t* __restrict__ kout,
-                                                                 uint32_t n, uint32_t num_buckets,
-                                                                 AnyF f,
-                                                                 uint32_t* __restrict__ start) {
-    const size_t stride = (size_t)gridDim.x * kThreads;
-    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride) {
-        const uint32_t hi = f(kout[i]);
-        const uint32_t lo = i == 0 ? 0u : f(kout[i - 1]) + 1u;
-        for (uint32_t b = lo; b <= hi && b < num_buckets; ++b) start[b] = (uint32_t)i;
-        if (i + 1 == n)
-            for (uint32_t b = hi + 1u; b < num_buckets; ++b) start[b] = n;
-    }
-}
-
 gms_status large_scratch_alloc(LargeScratch& L, uint32_t n_capacity, uint32_t num_buckets,
                                bool temporaries) {
     large_scratch_free(L);
@@ -84,10 +67,7 @@ gms_status run_large_multisplit(LargeScratch& L, Scratch& s, const uint32_t* kin
         sv = dv;
     }
     if (!d_offsets) return GMS_SUCCESS;
-    const uint32_t blocks = (uint32_t)std::min<size_t>(((size_t)n + kThreads - 1) / kThreads, 8192);
-    hipLaunchKernelGGL(bucket_starts_kernel, dim3(blocks), dim3(kThreads), 0, stream,
-                       (const uint32_t*)kout, n, B, base, d_offsets);
-    return hip_status(hipGetLastError());
+    return hip_status(launch_bucket_starts<1>(kout, n, B, base, d_offsets, stream));
 }
 
 }  // namespace gms

@@ -108,6 +108,8 @@ __global__ __launch_bounds__(kThreads) void scan_down_kernel(const uint32_t* in,
     }
 }
 
+// out[b] = G[b * num_tiles]: the first word of each bucket's row of the
+// scanned bucket-major matrix is the bucket's start.
 __global__ void gather_offsets_kernel(const uint32_t* __restrict__ G, uint32_t num_tiles,
                                       uint32_t num_buckets, uint32_t* __restrict__ out) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;

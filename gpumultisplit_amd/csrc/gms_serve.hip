@@ -270,9 +270,18 @@ struct gms_server {
                 break;
             }
             if (!inflight.empty()) break;  // retry once a step has retired
-            items.resize(items.size() / 2);  // should not happen: shares are reserved
+            // Nothing in flight: the arena holds only admitted sorts'
+            // buffers, which can leave the free space in pieces too small for
+            // the step's tables even though the reservations fit the budget.
+            // Try fewer items; a single item that still does not fit fails
+            // its request below instead of waiting for space that no
+            // retirement will free.
+            if (items.size() == 1) break;
+            items.resize(items.size() / 2);
             reqs.resize(items.size());
         }
+        keys = 0;
+        for (const auto& it : items) keys += it.n;
         if (s == GMS_SUCCESS && placed) {
             if (!events.empty()) {
                 ev = events.back();
@@ -293,7 +302,14 @@ struct gms_server {
             fail_all(s);
             return false;
         }
-        if (!placed) return false;
+        if (!placed) {
+            if (inflight.empty() && reqs.size() == 1) {
+                finish(reqs[0], GMS_ERR_CAPACITY);
+                cv_done.notify_all();
+                return true;  // progress: its buffers are free again
+            }
+            return false;
+        }
         for (auto& r : reqs) ++r->next;
         inflight.push_back(Step{std::move(reqs), off, bytes, ev});
         slot ^= 1;
@@ -321,6 +337,8 @@ struct gms_server {
             if (stopping && pending.empty() && active.empty() && inflight.empty()) break;
             if (!inflight.empty())
                 cv_sched.wait_for(lk, std::chrono::microseconds(20));
+            else if (!active.empty())
+                cv_sched.wait_for(lk, std::chrono::milliseconds(1));  // never parked with work left
             else
                 cv_sched.wait(lk);
         }
@@ -461,6 +479,21 @@ gms_status gms_server_wait(gms_server* s, uint64_t ticket, double* latency_us) {
     s->by_ticket.erase(ticket);
     return r->status;
 }
+
+#ifdef GMS_HOST_EMULATION
+// Test hooks of the CPU emulation build only (not in the GPU library):
+// occupy or free a range of the server's arena, to test how steps are placed
+// in a fragmented arena.  The caller makes sure the server is idle.
+gms_status gms_emu_server_hold(gms_server* s, size_t bytes, size_t* off) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    return s->arena.alloc(bytes, off) ? GMS_SUCCESS : GMS_ERR_ALLOC;
+}
+gms_status gms_emu_server_unhold(gms_server* s, size_t off, size_t bytes) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->arena.release(off, bytes);
+    return GMS_SUCCESS;
+}
+#endif
 
 gms_status gms_server_get_stats(gms_server* s, gms_server_stats* out) {
     if (!s || !out) return GMS_ERR_INVALID_ARG;

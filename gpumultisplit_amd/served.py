@@ -7,7 +7,10 @@ Request mix: each request is a multisplit with B in {2, 16, 64, 256} buckets
 keys.  Keys are generated on the device, so inputs are resident in HBM
 before the first request arrives.  ``run_open_loop`` submits requests at
 given arrival times (open loop: arrivals do not wait for completions) and
-collects the server's submit-to-completion latencies.
+collects the server's submit-to-completion latencies; ``run_steady`` replays
+the request set at a Poisson rate for a stated duration, rotating the
+requests over several copies of their output buffers so that no two
+requests in flight write the same memory.
 """
 from __future__ import annotations
 
@@ -85,6 +88,71 @@ def run_open_loop(server, recs, arrivals_s):
         tickets.append(server.submit(rec["req"]))
     lats = [server.wait(t) for t in tickets]
     return np.array(lats), time.perf_counter() - t0
+
+
+def output_copies(gms, torch, recs, copies: int):
+    """`copies` sets of the requests: the first is recs itself, the others
+    read the same inputs and write their own output buffers."""
+    sets = [recs]
+    for _ in range(copies - 1):
+        out = []
+        for r in recs:
+            s = r["spec"]
+            kout = torch.empty_like(r["kin"])
+            vout = torch.empty_like(r["kin"]) if s["kv"] else None
+            offs = torch.empty_like(r["offs"]) if r["offs"] is not None else None
+            if s["op"] == "sort":
+                req = gms.sort_request(r["kin"], kout, s["n"], 8, d_value_in=r["vin"],
+                                       d_value_out=vout)
+            else:
+                req = gms.multisplit_request(r["kin"], kout, s["n"], s["B"], gms.range_bucket(s["B"]),
+                                             d_value_in=r["vin"], d_value_out=vout, d_offsets=offs)
+            out.append({"spec": s, "req": req, "kin": r["kin"], "kout": kout, "vin": r["vin"],
+                        "vout": vout, "offs": offs})
+        sets.append(out)
+    torch.cuda.synchronize()
+    return sets
+
+
+def run_steady(server, rec_sets, rate_per_s: float, duration_s: float, seed: int):
+    """Open-loop Poisson arrivals at rate_per_s for duration_s (at least one
+    pass over the request set): request k is rec_sets[(k // R) % C][k % R].
+    Before a request's buffers are used again, the previous request on them
+    must have completed; the number of times that had to be waited for is
+    returned (0 when the copies keep uses apart).  Returns a dict with the
+    latencies (us), the wall time from the first submit to the last
+    completion, the submitted count and total keys."""
+    R, C = len(rec_sets[0]), len(rec_sets)
+    count = max(R, int(round(rate_per_s * duration_s)))
+    arr = poisson_arrivals(count, rate_per_s, seed)
+    last = {}       # (copy, index) -> ticket of the last request on those buffers
+    lat_of = {}     # ticket -> latency, for tickets already waited for
+    reuse_waits = 0
+    tickets = []
+    keys = 0
+    t0 = time.perf_counter()
+    for k in range(count):
+        i, c = k % R, (k // R) % C
+        prev = last.get((c, i))
+        if prev is not None and prev not in lat_of:
+            done, lat = server.poll(prev)
+            if not done:
+                reuse_waits += 1
+            lat_of[prev] = server.wait(prev)
+        target = t0 + arr[k]
+        while time.perf_counter() < target:
+            pass
+        t = server.submit(rec_sets[c][i]["req"])
+        last[(c, i)] = t
+        tickets.append(t)
+        keys += rec_sets[c][i]["spec"]["n"]
+    for t in tickets:
+        if t not in lat_of:
+            lat_of[t] = server.wait(t)
+    wall = time.perf_counter() - t0
+    return {"lat_us": np.array([lat_of[t] for t in tickets]), "wall_s": wall,
+            "requests": count, "keys": keys, "reuse_waits": reuse_waits,
+            "arrival_span_s": float(arr[-1])}
 
 
 def poisson_arrivals(count: int, rate_per_s: float, seed: int):

@@ -211,6 +211,48 @@ gms_status gms_multisplit_prescan_matrix(gms_multisplit_ctx *ctx, const uint32_t
 gms_status gms_exclusive_scan_u32(gms_multisplit_ctx *ctx, const uint32_t *d_in,
                                   uint32_t *d_out, uint32_t n, void *stream);
 
+/* Stages of a multisplit whose bucket identifier is the caller's own functor.
+ * The reference takes any `bucket_t` functor object (src/api/multisplit.cuh:
+ * 81-88, :199-208, compaction.cuh:525-573; "provided by the programmer",
+ * README.md:4), and a functor cannot cross a C ABI: the header-only
+ * gpumultisplit_device.hpp, compiled by the caller's hipcc, instantiates this
+ * library's prescan and postscan kernel templates with the functor and
+ * launches them on the scratch a plan describes; these calls do the rest.
+ *   gms_multisplit_stage_plan  validate the call as gms_multisplit_wide does
+ *                              and describe its stages (or finish it: n == 0
+ *                              or one bucket, `done` = 1);
+ *   (caller: prescan)          gms_stage_scan  (caller: postscan)
+ *   gms_stage_offsets          bucket starts of the finished call.
+ * 32-bit elements up to 1024 buckets, elements with a 64-bit key or value up
+ * to 256 (GMS_ERR_UNSUPPORTED_BUCKETS above). */
+typedef struct gms_stage_plan {
+    uint32_t num_buckets;
+    uint32_t log_b;          /* the count matrices have 2^log_b rows */
+    uint32_t key_words;      /* 32-bit words per key (1 or 2) */
+    uint32_t value_words;    /* 32-bit words per value (0, 1 or 2) */
+    uint32_t n;
+    uint32_t num_tiles;      /* tiles of the kernels' tile shape */
+    uint32_t large;          /* n takes the long-tile shape (32-bit elements) */
+    uint32_t vec;            /* the inputs are 16-byte aligned */
+    uint32_t done;           /* the library finished the call: nothing to launch */
+    uint32_t *d_matrix;      /* per-tile bucket counts, scanned in place */
+    uint32_t *d_partials;    /* scan scratch */
+    uint32_t *d_staging;     /* num_buckets words of the context (bucket starts) */
+    uint64_t scan_words;     /* words the scan stage covers */
+} gms_stage_plan;
+
+gms_status gms_multisplit_stage_plan(gms_multisplit_ctx *ctx, const void *d_key_in,
+                                     const void *d_value_in, void *d_key_out,
+                                     void *d_value_out, uint32_t n, uint32_t key_bytes,
+                                     uint32_t value_bytes, void *stream, gms_stage_plan *plan);
+
+/* The scan stage between the caller's prescan and postscan. */
+gms_status gms_stage_scan(const gms_stage_plan *plan, void *stream);
+
+/* After the postscan: the num_buckets bucket starts to device memory d_out
+ * (asynchronous; plan->d_staging is a valid d_out). */
+gms_status gms_stage_offsets(const gms_stage_plan *plan, uint32_t *d_out, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Multisplit radix sort                                                     */
 /* ------------------------------------------------------------------------ */
@@ -290,6 +332,15 @@ gms_status gms_compaction_wide(gms_compaction_ctx *ctx, const void *d_key_in,
                                gms_bucket_desc64 desc, uint32_t *h_count, uint32_t *d_count,
                                void *stream);
 
+/* gms_multisplit_stage_plan for a compaction with the caller's own 2-bucket
+ * functor (compaction_key_only / _key_value with any bucket_t,
+ * multisplit2_compaction.cuh:525-573): after the postscan, the bucket-0
+ * count is the start of bucket 1 (gms_stage_offsets). */
+gms_status gms_compaction_stage_plan(gms_compaction_ctx *ctx, const void *d_key_in,
+                                     const void *d_value_in, void *d_key_out,
+                                     void *d_value_out, uint32_t n, uint32_t key_bytes,
+                                     uint32_t value_bytes, void *stream, gms_stage_plan *plan);
+
 /* ------------------------------------------------------------------------ */
 /* Histogram                                                                 */
 /* ------------------------------------------------------------------------ */
@@ -367,6 +418,29 @@ gms_status gms_batch_destroy(gms_batch *b);
  * stay below 2^32.  Drops any captured graph.  Buffers are not touched. */
 gms_status gms_batch_plan(gms_batch *b, const gms_batch_item *items, uint32_t count,
                           void *stream);
+
+/* One independent radix sort of a batch: multisplit_sort_{4..8}bit[_pairs]
+ * (src/api/multisplit_sort.cuh:320-559, multisplit_sort_pairs.cuh:21-260)
+ * with the same digit schedule, as gms_sort; values NULL = keys only.  The
+ * outputs double as a pass buffer and the input is never written. */
+typedef struct gms_batch_sort_item {
+    const uint32_t *d_key_in;
+    const uint32_t *d_value_in;
+    uint32_t *d_key_out;
+    uint32_t *d_value_out;
+    uint32_t n;
+    uint32_t digit_bits;    /* 4, 5, 6, 7 or 8 */
+} gms_batch_sort_item;
+
+/* Plans multisplits and radix sorts as one batch: step 0 runs every multisplit
+ * and the first digit pass of every sort as one batched multisplit, step p
+ * the p-th pass of every sort that has one, in stream order.  The batch owns
+ * the sorts' other pass buffer (4 bytes per key, 8 with values).
+ * gms_batch_run, _capture, _launch_graph, _autotune and _stats cover every
+ * step.  gms_batch_plan is this call with no sorts. */
+gms_status gms_batch_plan_mixed(gms_batch *b, const gms_batch_item *items, uint32_t count,
+                                const gms_batch_sort_item *sorts, uint32_t num_sorts,
+                                void *stream);
 
 /* Runs every planned multisplit: one prescan and one postscan launch per
  * (bucket class, key-value) group and one scan, whatever the item count. */

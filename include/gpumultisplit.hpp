@@ -16,10 +16,13 @@
 //    reference's functors are provided (ms_bucket_sort_Nbit at global scope
 //    as in multisplit_sort.cuh; delta_func, identity_bucket, test_bucket --
 //    which the reference defines in its drivers -- and range_bucket in
-//    namespace gms).  A caller's own functor type is accepted after
-//    specialising gms::bucket_traits<T> with
+//    namespace gms) and run the library's precompiled kernels.  Any other
+//    functor type (a hash, a table lookup, ...) runs as device code when the
+//    caller includes gpumultisplit_device.hpp instead of this header and
+//    compiles with hipcc: multisplits and compactions then instantiate the
+//    library's kernel templates with it.  Alternatively a functor equal to a
+//    built-in kind can specialise gms::bucket_traits<T> with
 //      static gms_bucket_desc desc(const T&, uint32_t num_buckets);
-//    mapping it to one of the built-in kinds.
 //  * KeyT and ValueT of multisplits, sorts, compactions and functor
 //    histograms may be 32- or 64-bit (any 8-byte operand takes the 64-bit
 //    kernels).  A 64-bit identifier parameter (a delta above 2^32) needs
@@ -86,11 +89,20 @@ struct range_bucket {
     }
 };
 
+// No descriptor: the functor runs as device code (gpumultisplit_device.hpp).
 template <class T, class Enable = void>
-struct bucket_traits {
-    static_assert(sizeof(T) == 0,
-                  "no gms_bucket_desc for this functor type: specialise gms::bucket_traits");
-};
+struct bucket_traits {};
+
+template <class T, class = void>
+struct has_desc : std::false_type {};
+template <class T>
+struct has_desc<T, decltype((void)bucket_traits<T>::desc(std::declval<const T&>(), 0u))>
+    : std::true_type {};
+template <class F>
+constexpr bool has_desc_v = has_desc<typename std::decay<F>::type>::value;
+
+template <class T>
+struct always_false : std::false_type {};
 
 template <>
 struct bucket_traits<delta_func> {
@@ -121,6 +133,8 @@ struct bucket_traits<range_bucket> {
 
 template <class F>
 inline gms_bucket_desc desc_of(const F& f, uint32_t num_buckets) {
+    static_assert(has_desc_v<F>,
+                  "this call takes a built-in identifier (a functor with a gms_bucket_desc)");
     return bucket_traits<typename std::decay<F>::type>::desc(f, num_buckets);
 }
 
@@ -135,6 +149,8 @@ struct has_desc64<T, decltype((void)bucket_traits<T>::desc64(std::declval<const 
 template <class F>
 inline gms_bucket_desc64 desc64_of(const F& f, uint32_t num_buckets) {
     using T = typename std::decay<F>::type;
+    static_assert(has_desc_v<F>,
+                  "this call takes a built-in identifier (a functor with a gms_bucket_desc)");
     if constexpr (has_desc64<T>::value) {
         return bucket_traits<T>::desc64(f, num_buckets);
     } else {
@@ -234,7 +250,18 @@ template <typename KeyT, typename bucket_t>
 inline void multisplit_key_only(KeyT* d_key_in, KeyT* d_key_out, uint32_t num_elements,
                                 multisplit_context& context, bucket_t bucket_identifier,
                                 uint32_t* h_offsets = nullptr, void* stream = nullptr) {
-    if constexpr (gms::elem_bytes<KeyT>() == 8) {
+    if constexpr (!gms::has_desc_v<bucket_t>) {
+#ifdef GMS_DEVICE_FUNCTORS
+        gms::check(gms::device::multisplit(context.handle, d_key_in, (const KeyT*)nullptr,
+                                           d_key_out, (KeyT*)nullptr, num_elements,
+                                           bucket_identifier, h_offsets, stream),
+                   "multisplit_key_only");
+#else
+        static_assert(gms::always_false<bucket_t>::value,
+                      "this functor has no gms_bucket_desc: include gpumultisplit_device.hpp "
+                      "(compiled with hipcc) to run it as device code");
+#endif
+    } else if constexpr (gms::elem_bytes<KeyT>() == 8) {
         gms::check(gms_multisplit_wide(context.handle, d_key_in, nullptr, d_key_out, nullptr,
                                        num_elements, 8, 0,
                                        gms::desc64_of(bucket_identifier, context.num_buckets),
@@ -256,7 +283,19 @@ inline void multisplit_key_value(KeyT* d_key_in, ValueT* d_value_in, KeyT* d_key
                                  multisplit_context& context, bucket_t bucket_identifier,
                                  uint32_t* h_offsets = nullptr, void* stream = nullptr) {
     constexpr uint32_t kb = gms::elem_bytes<KeyT>(), vb = gms::elem_bytes<ValueT>();
-    if constexpr (kb == 8 || vb == 8) {
+    if constexpr (!gms::has_desc_v<bucket_t>) {
+#ifdef GMS_DEVICE_FUNCTORS
+        if (num_elements && !d_value_in) throw gms::error(GMS_ERR_INVALID_ARG, "multisplit_key_value");
+        gms::check(gms::device::multisplit(context.handle, d_key_in, d_value_in, d_key_out,
+                                           d_value_out, num_elements, bucket_identifier, h_offsets,
+                                           stream),
+                   "multisplit_key_value");
+#else
+        static_assert(gms::always_false<bucket_t>::value,
+                      "this functor has no gms_bucket_desc: include gpumultisplit_device.hpp "
+                      "(compiled with hipcc) to run it as device code");
+#endif
+    } else if constexpr (kb == 8 || vb == 8) {
         gms::check(gms_multisplit_wide(context.handle, d_key_in, d_value_in, d_key_out, d_value_out,
                                        num_elements, kb, vb,
                                        gms::desc64_of(bucket_identifier, context.num_buckets),
@@ -401,7 +440,18 @@ inline unsigned compaction_key_only(key_t* d_key_in, key_t* d_key_out, uint32_t 
                                     compaction_context& context, bucket_t bucket_identifier,
                                     void* stream = nullptr) {
     unsigned count = 0;
-    if constexpr (gms::elem_bytes<key_t>() == 8) {
+    if constexpr (!gms::has_desc_v<bucket_t>) {
+#ifdef GMS_DEVICE_FUNCTORS
+        gms::check(gms::device::compaction(context.handle, d_key_in, (const key_t*)nullptr,
+                                           d_key_out, (key_t*)nullptr, num_elements,
+                                           bucket_identifier, &count, stream),
+                   "compaction_key_only");
+#else
+        static_assert(gms::always_false<bucket_t>::value,
+                      "this functor has no gms_bucket_desc: include gpumultisplit_device.hpp "
+                      "(compiled with hipcc) to run it as device code");
+#endif
+    } else if constexpr (gms::elem_bytes<key_t>() == 8) {
         gms::check(gms_compaction_wide(context.handle, d_key_in, nullptr, d_key_out, nullptr,
                                        num_elements, 8, 0, gms::desc64_of(bucket_identifier, 2),
                                        &count, nullptr, stream),
@@ -423,7 +473,19 @@ inline unsigned compaction_key_value(key_t* d_key_in, value_t* d_value_in, key_t
                                      void* stream = nullptr) {
     unsigned count = 0;
     constexpr uint32_t kb = gms::elem_bytes<key_t>(), vb = gms::elem_bytes<value_t>();
-    if constexpr (kb == 8 || vb == 8) {
+    if constexpr (!gms::has_desc_v<bucket_t>) {
+#ifdef GMS_DEVICE_FUNCTORS
+        if (num_elements && !d_value_in) throw gms::error(GMS_ERR_INVALID_ARG, "compaction_key_value");
+        gms::check(gms::device::compaction(context.handle, d_key_in, d_value_in, d_key_out,
+                                           d_value_out, num_elements, bucket_identifier, &count,
+                                           stream),
+                   "compaction_key_value");
+#else
+        static_assert(gms::always_false<bucket_t>::value,
+                      "this functor has no gms_bucket_desc: include gpumultisplit_device.hpp "
+                      "(compiled with hipcc) to run it as device code");
+#endif
+    } else if constexpr (kb == 8 || vb == 8) {
         gms::check(gms_compaction_wide(context.handle, d_key_in, d_value_in, d_key_out, d_value_out,
                                        num_elements, kb, vb, gms::desc64_of(bucket_identifier, 2),
                                        &count, nullptr, stream),

@@ -76,6 +76,13 @@ def lib():
         L.ms_oracle_tile_histograms.argtypes = [ctypes.c_uint32] * 3 + [u32p, ctypes.c_uint64,
                                                                         ctypes.c_uint32, u32p]
         L.ms_oracle_tile_histograms.restype = None
+        L.ms_baseline_stable_sort.argtypes = [u32p] * 4 + [ctypes.c_uint64]
+        L.ms_baseline_stable_sort.restype = None
+        L.ms_baseline_histogram.argtypes = [u32p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32,
+                                            u32p]
+        L.ms_baseline_histogram.restype = None
+        L.ms_baseline_stable_partition.argtypes = [u32p] * 4 + [ctypes.c_uint64, ctypes.c_uint64]
+        L.ms_baseline_stable_partition.restype = ctypes.c_uint64
         _lib = L
     return _lib
 
@@ -221,3 +228,36 @@ def sort_wide(keys, digit_bits: int = 8, values=None):
     assert rc == 0
     vout = None if vo is None else vo[:vw.size].view(np.uint64 if nvw == 2 else np.uint32)
     return ko[:kw.size].view(np.uint64), vout
+
+
+# ---------------------------------------------------------------------------
+# Standard-library CPU baselines (oracle/baseline.cpp): what bench.py times on
+# the host for the sort, histogram and compaction rows.
+# ---------------------------------------------------------------------------
+def baseline_stable_sort(keys, values=None):
+    """std::stable_sort of (key, value) pairs by key (keys alone if no values)."""
+    keys = _u32(keys)
+    ko = np.empty_like(keys)
+    vals = None if values is None else _u32(values)
+    vo = None if vals is None else np.empty_like(vals)
+    lib().ms_baseline_stable_sort(_ptr(keys), _ptr(vals), _ptr(ko), _ptr(vo), keys.size)
+    return ko, vo
+
+
+def baseline_histogram(keys, bins: int = 256, shift: int = 0) -> np.ndarray:
+    """Counting loop over (key >> shift) & (bins - 1)."""
+    keys = _u32(keys)
+    h = np.zeros(bins, dtype=np.uint32)
+    lib().ms_baseline_histogram(_ptr(keys), keys.size, shift, bins, _ptr(h))
+    return h
+
+
+def baseline_stable_partition(keys, threshold: int, values=None):
+    """std::stable_partition by key < threshold.  Returns (count, keys, values)."""
+    keys = _u32(keys)
+    ko = np.empty_like(keys)
+    vals = None if values is None else _u32(values)
+    vo = None if vals is None else np.empty_like(vals)
+    c = lib().ms_baseline_stable_partition(_ptr(keys), _ptr(vals), _ptr(ko), _ptr(vo), keys.size,
+                                           threshold)
+    return int(c), ko, vo

@@ -328,6 +328,33 @@ int main(int argc, char** argv) {
             }
         }
     }
+    // 64-bit elements into more than 256 buckets on one context whose
+    // capacity grows between calls: the pass temporaries of the first call
+    // must not be reused at the larger size (ASan sees an overrun if they are).
+    {
+        const uint32_t B = 1024, n1 = 1000, n2 = 3072u * 3 + 11;
+        gms_multisplit_ctx* c;
+        gms_multisplit_create(B, &c);
+        for (uint32_t n : {n1, n2}) {
+            EXPECT(gms_multisplit_allocate(c, n, 1) == GMS_SUCCESS, "allocate %u", n);
+            Buf kin(n * 2), vin(n * 2), kout(n * 2), vout(n * 2);
+            for (uint32_t i = 0; i < n * 2; ++i) {
+                kin.p[i] = rnd();
+                vin.p[i] = i;
+            }
+            std::vector<uint32_t> offs(B), wk(n * 2), wv(n * 2), wo(B);
+            EXPECT(gms_multisplit_wide(c, kin.p, vin.p, kout.p, vout.p, n, 8, 8,
+                                       gms_bucket_desc64{GMS_BUCKET_RANGE, 0, 0}, nullptr,
+                                       offs.data(), nullptr) == GMS_SUCCESS,
+                   "wide regrow status n=%u", n);
+            ms_oracle_multisplit_wide(0, B, 0, 0, 2, 2, kin.p, vin.p, wk.data(), wv.data(), n,
+                                      wo.data());
+            EXPECT(memcmp(kout.p, wk.data(), (size_t)n * 8) == 0 &&
+                       memcmp(vout.p, wv.data(), (size_t)n * 8) == 0 && offs == wo,
+                   "wide regrow n=%u", n);
+        }
+        gms_multisplit_destroy(c);
+    }
     printf("wide done\n");
 
     // wave-level variant: exact-size buffers, ragged subproblems

@@ -240,3 +240,22 @@ def test_wide_entry_points_validate_before_any_hip_call():
     assert L.gms_sort_allocate_wide(s, 10, 3, 8, 0) == 1  # digit width outside 4..8
     assert L.gms_sort_allocate_wide(s, 10, 8, 8, 2) == 1
     assert L.gms_sort_destroy(s) == 0
+
+
+def test_strided_and_expanded_buffers_rejected_before_any_launch():
+    """Kernels take a buffer's address and read n consecutive elements, so a
+    strided view or an expanded tensor is refused on the host, before the
+    library is called (this runs without a GPU)."""
+    import torch
+    base = torch.zeros(64, dtype=torch.int32)
+    ctx = gms.multisplit_context(4)
+    for bad in (base[::2], base.view(8, 8)[:, 0], torch.zeros(1, dtype=torch.int32).expand(32)):
+        assert not bad.is_contiguous()
+        with pytest.raises(ValueError, match="contiguous"):
+            gms.multisplit_key_only(bad, torch.zeros(32, dtype=torch.int32), bad.numel(), ctx,
+                                    gms.range_bucket(4))
+        with pytest.raises(ValueError, match="contiguous"):
+            gms._ptr(bad)
+    offs = torch.zeros(8, dtype=torch.int32)[::2]
+    with pytest.raises(ValueError, match="contiguous"):
+        gms._offsets_buffer(offs, 4)

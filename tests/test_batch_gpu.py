@@ -158,6 +158,103 @@ def test_batch_validation(dev):
     gms.Batch([gms.BatchItem(a, x[:64], 64, 4), gms.BatchItem(a, x[64:128], 64, 4)])  # shared input ok
 
 
+# (n, digit_bits, kv) of the radix sorts planned beside multisplits
+SORTS = [(0, 8, True), (1, 4, False), (100_003, 8, True), (70_001, 5, False), (6144, 6, True),
+         (300_000, 7, True), (1 << 20, 4, True), (50_000, 8, False)]
+
+
+def _make_sorts(dev, seed):
+    sorts, want, inputs = [], [], []
+    for i, (n, bits, kv) in enumerate(SORTS):
+        keys = W.uniform_keys(n, seed=seed + i)
+        if i == 3:
+            keys = keys & np.uint32(0xFFF)       # few distinct keys: stability across passes
+        vals = (np.arange(n, dtype=np.uint32) * np.uint32(5) + np.uint32(i)).astype(np.uint32)
+        kin = to_dev(keys, dev) if n else filled_u32(0, dev)
+        vin = (to_dev(vals, dev) if n else filled_u32(0, dev)) if kv else None
+        s = gms.BatchSortItem(kin, filled_u32(n, dev), n, bits, d_value_in=vin,
+                              d_value_out=filled_u32(n, dev) if kv else None)
+        sorts.append(s)
+        inputs.append((keys, vals if kv else None))
+        want.append(oracle.sort(keys, bits, vals if kv else None))
+    return sorts, want, inputs
+
+
+def _check_sorts(sorts, want, inputs, label):
+    for i, (s, (wk, wv), (keys, vals)) in enumerate(zip(sorts, want, inputs)):
+        n = s.n
+        np.testing.assert_array_equal(from_dev(s.d_key_in)[:n], keys, err_msg=f"{label} sort {i} input")
+        np.testing.assert_array_equal(from_dev(s.d_key_out)[:n], wk, err_msg=f"{label} sort {i} keys")
+        if vals is not None:
+            np.testing.assert_array_equal(from_dev(s.d_value_in)[:n], vals,
+                                          err_msg=f"{label} sort {i} value input")
+            np.testing.assert_array_equal(from_dev(s.d_value_out)[:n], wv,
+                                          err_msg=f"{label} sort {i} values")
+
+
+def _clear_sorts(sorts):
+    for s in sorts:
+        s.d_key_out.fill_(0x5A5A5A5A)
+        if s.d_value_out is not None:
+            s.d_value_out.fill_(0x5A5A5A5A)
+
+
+@pytest.mark.parametrize("tuning", [(8, 0), (16, 1), (24, 0), (24, 1), (32, 0)])
+def test_batch_sorts_and_multisplits_match_oracle(dev, tuning):
+    """Sorts as passes of the batch beside multisplits: every sort equals the
+    oracle's sort with the same digit schedule, every multisplit the oracle's
+    multisplit, inputs unchanged; a second run on the same plan agrees."""
+    import torch
+    banner(f"batch sorts tuning={tuning}")
+    items, want = _make(dev, MIXED, seed=23)
+    sorts, swant, inputs = _make_sorts(dev, seed=41)
+    b = gms.Batch(items, sorts=sorts)
+    b.tuning = {"items_per_thread": tuning[0], "persistent": tuning[1]}
+    b.run()
+    torch.cuda.synchronize()
+    _check(items, want, f"run {tuning}")
+    _check_sorts(sorts, swant, inputs, f"run {tuning}")
+    _clear(items)
+    _clear_sorts(sorts)
+    b.run()
+    torch.cuda.synchronize()
+    _check(items, want, f"rerun {tuning}")
+    _check_sorts(sorts, swant, inputs, f"rerun {tuning}")
+    passes = sum(len(oracle.sort_schedule(bits)) * n for n, bits, _ in SORTS)
+    assert b.stats()["keys"] == passes + sum(c[0] for c in MIXED)
+
+
+def test_batch_sorts_only_graph_replays_and_autotune(dev):
+    """A batch of sorts only: graph replays with eager work between them, and
+    the autotuner's choice, each equal the oracle."""
+    import torch
+    banner("batch sorts only: graph + autotune")
+    sorts, swant, inputs = _make_sorts(dev, seed=3)
+    b = gms.Batch([], sorts=sorts)
+    b.capture()
+    for rep in range(3):
+        _clear_sorts(sorts)
+        torch.cuda.synchronize()
+        b.replay()
+        torch.cuda.synchronize()
+        _check_sorts(sorts, swant, inputs, f"replay {rep}")
+    t = b.autotune(reps=1)
+    assert t["ms_per_run"] > 0
+    _clear_sorts(sorts)
+    b.run()
+    torch.cuda.synchronize()
+    _check_sorts(sorts, swant, inputs, "autotuned")
+
+
+def test_batch_sort_validation(dev):
+    a = filled_u32(64, dev)
+    o = filled_u32(64, dev)
+    with pytest.raises(gms.GmsError):          # 3-bit digits
+        gms.Batch([], sorts=[gms.BatchSortItem(a, o, 64, 3)])
+    with pytest.raises(gms.GmsError):          # sort output overlaps a multisplit's input
+        gms.Batch([gms.BatchItem(a, o, 64, 4)], sorts=[gms.BatchSortItem(o, a, 64, 8)])
+
+
 @pytest.mark.slow
 def test_batch_4096_loguniform_equals_plain_calls(dev):
     """The batched workload: 4096 key-value multisplits, sizes round(2^U),

@@ -42,3 +42,28 @@ def test_cpp_api_driver_on_gpu(dev, n):
     print(f"[case] api_driver n={n}", flush=True)
     r = subprocess.run([exe, str(n)], capture_output=True, text=True, timeout=110)
     assert r.returncode == 0 and "API DRIVER OK" in r.stdout, r.stdout + r.stderr
+
+
+def test_functor_driver_on_emulation():
+    """Programmer-supplied functors through gpumultisplit_device.hpp on the
+    CPU emulation of the kernels: multiplicative hash (2..1024 buckets), ids
+    past the bucket count (clamped), popcount mod B (any B) and its parity as
+    a compaction predicate, 64-bit keys/values with a 64-bit hash; every
+    result against std::stable_sort of the positions by host-computed ids."""
+    subprocess.run(["make", "-s", "-C", CPP, "bin/functor_driver_host"], check=True)
+    r = subprocess.run([os.path.join(CPP, "bin", "functor_driver_host"), "0", "1", "6143", "6145",
+                        "20000"], capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0 and "FUNCTOR DRIVER OK" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", ["0 1 6143 6144 6145 12289", "100003", str((1 << 22) + 7)])
+def test_functor_driver_on_gpu(dev, n):
+    """The same functor cases on the GPU: the driver's own instantiations of
+    the library's kernel templates (compiled by hipcc in tests/cpp), linked
+    to libgpumultisplit.so for the stages it runs."""
+    exe = os.path.join(CPP, "bin", "functor_driver")
+    assert os.path.exists(exe), "tests/cpp/bin/functor_driver is not built"
+    print(f"[case] functor_driver n={n}", flush=True)
+    r = subprocess.run([exe] + n.split(), capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0 and "FUNCTOR DRIVER OK" in r.stdout, r.stdout + r.stderr

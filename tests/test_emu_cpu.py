@@ -7,6 +7,7 @@ tests pin the compiled gfx950 code.  The AddressSanitizer build of the same
 sources is `make -C tests/emu emu_check_asan && tests/emu/emu_check_asan`.
 """
 import ctypes
+import fcntl
 import os
 import subprocess
 
@@ -24,7 +25,10 @@ EMU_LIB = os.path.join(EMU_DIR, "libgpumultisplit_emu.so")
 
 @pytest.fixture(scope="module")
 def emu():
-    subprocess.run(["make", "-s", "-C", EMU_DIR, "libgpumultisplit_emu.so"], check=True)
+    # one build at a time when pytest-xdist workers start together
+    with open(os.path.join(EMU_DIR, ".build.lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        subprocess.run(["make", "-s", "-C", EMU_DIR, "libgpumultisplit_emu.so"], check=True)
     L = ctypes.CDLL(EMU_LIB)
     for name, res, args in gms._SIGS:
         fn = getattr(L, name)
@@ -71,6 +75,23 @@ def test_emulated_multisplit_matches_oracle(emu, B, kv):
         np.testing.assert_array_equal(got[2], wo)
         if kv:
             np.testing.assert_array_equal(got[1], wv)
+
+
+@pytest.mark.parametrize("B,kv,extra", [(2, True, 0), (8, False, 5), (64, True, 777),
+                                        (256, True, 6143), (256, False, 1)])
+def test_emulated_many_tiles_ragged_last(emu, B, kv, extra):
+    """19 full tiles and a ragged (or absent) last tile, on zipf keys where
+    whole tiles fall into one bucket."""
+    tile = gms.tile_keys(B, kv, 0)
+    n = 19 * tile + extra
+    keys = W.keys_for("zipf", n, B, seed=B + extra) if B >= 8 else W.uniform_keys(n, seed=B)
+    got = emu_multisplit(emu, keys, B, gms.range_bucket(B), kv)
+    wk, wv, wo = oracle.multisplit(keys, B, oracle.RANGE, 0,
+                                   np.arange(n, dtype=np.uint32) if kv else None)
+    np.testing.assert_array_equal(got[0], wk)
+    np.testing.assert_array_equal(got[2], wo)
+    if kv:
+        np.testing.assert_array_equal(got[1], wv)
 
 
 def test_emulated_skewed_and_delta(emu):
@@ -179,6 +200,89 @@ def test_emulated_batch_rejects_overlapping_outputs(emu):
     bad = gms.gms_batch_item(p(a), None, p(b), None, 32, 300, gms.range_bucket(4).desc(), None)
     assert emu.gms_batch_plan(h, (gms.gms_batch_item * 1)(bad), 1, None) == 2
     emu.gms_batch_destroy(h)
+
+
+# (n, digit_bits, kv) of the radix sorts planned beside multisplits
+SORT_CASES = [(0, 8, True), (1, 4, False), (3000, 8, True), (4099, 5, False), (777, 6, True),
+              (6145, 7, True), (2048, 4, True)]
+
+
+@pytest.mark.parametrize("with_items,tuning", [(True, (24, 0)), (True, (8, 1)), (False, (16, 0))])
+def test_emulated_batch_with_sorts(emu, with_items, tuning):
+    """Radix sorts as passes of the batch (step 0: the multisplits and every
+    sort's first pass, step p: the p-th passes) equal the oracle's sort, the
+    multisplits beside them equal the oracle, and no input is written."""
+    keep, sorts, want = [], [], []
+    for i, (n, bits, kv) in enumerate(SORT_CASES):
+        keys = W.uniform_keys(n, seed=70 + i)
+        if i == 3:
+            keys = keys & np.uint32(0x3FF)          # many equal digits: stability
+        vals = np.arange(n, dtype=np.uint32) + np.uint32(7 * i)
+        kout = np.full(max(n, 1), 0x5A5A5A5A, np.uint32)
+        vout = np.full(max(n, 1), 0x5A5A5A5A, np.uint32)
+        keep.append((keys, keys.copy(), vals, kout, vout))
+        sorts.append(gms.gms_batch_sort_item(p(keys) if n else None, p(vals) if kv else None,
+                                             p(kout), p(vout) if kv else None, n, bits))
+        want.append(oracle.sort(keys, bits, vals if kv else None))
+    cases = BATCH_CASES[:6] if with_items else []
+    ikeep, items, iwant = [], [], []
+    for i, (n, B, f, kind, param, kv) in enumerate(cases):
+        keys = W.uniform_keys(n, seed=90 + i)
+        vals = np.arange(n, dtype=np.uint32)
+        kout = np.full(max(n, 1), 0x5A5A5A5A, np.uint32)
+        vout = np.full(max(n, 1), 0x5A5A5A5A, np.uint32)
+        offs = np.full(B, 0xFFFFFFFF, np.uint32)
+        ikeep.append((keys, vals, kout, vout, offs))
+        items.append(gms.gms_batch_item(p(keys) if n else None, p(vals) if kv else None, p(kout),
+                                        p(vout) if kv else None, n, B, f.desc(), p(offs)))
+        iwant.append(oracle.multisplit(keys, B, kind, param, vals if kv else None))
+    h = ctypes.c_void_p()
+    assert emu.gms_batch_create(ctypes.byref(h)) == 0
+    try:
+        assert emu.gms_batch_set_tuning(h, gms.gms_batch_tuning(*tuning)) == 0
+        iarr = (gms.gms_batch_item * max(len(items), 1))(*items)
+        sarr = (gms.gms_batch_sort_item * len(sorts))(*sorts)
+        assert emu.gms_batch_plan_mixed(h, iarr, len(items), sarr, len(sorts), None) == 0
+        assert emu.gms_batch_run(h, None) == 0
+        k, m = ctypes.c_uint64(), ctypes.c_uint64()
+        w, lc = ctypes.c_uint32(), ctypes.c_uint32()
+        assert emu.gms_batch_stats(h, ctypes.byref(k), ctypes.byref(w), ctypes.byref(m),
+                                   ctypes.byref(lc)) == 0
+        passes = sum(len(oracle.sort_schedule(b)) * n for n, b, _ in SORT_CASES)
+        assert k.value == passes + sum(c[0] for c in cases)
+    finally:
+        emu.gms_batch_destroy(h)
+    for i, ((n, bits, kv), (keys, keys0, vals, kout, vout), (wk, wv)) in enumerate(
+            zip(SORT_CASES, keep, want)):
+        np.testing.assert_array_equal(keys, keys0, err_msg=f"sort {i} input written")
+        np.testing.assert_array_equal(kout[:n], wk, err_msg=f"sort {i} keys")
+        if kv:
+            np.testing.assert_array_equal(vout[:n], wv, err_msg=f"sort {i} values")
+    for i, ((keys, vals, kout, vout, offs), (wk, wv, wo)) in enumerate(zip(ikeep, iwant)):
+        n = keys.size
+        np.testing.assert_array_equal(kout[:n], wk, err_msg=f"item {i} keys")
+        np.testing.assert_array_equal(offs, wo, err_msg=f"item {i} offsets")
+        if cases[i][5]:
+            np.testing.assert_array_equal(vout[:n], wv, err_msg=f"item {i} values")
+
+
+def test_emulated_batch_sort_validation(emu):
+    a = np.zeros(64, np.uint32)
+    b = np.zeros(64, np.uint32)
+    h = ctypes.c_void_p()
+    assert emu.gms_batch_create(ctypes.byref(h)) == 0
+    try:
+        def plan(*sorts):
+            arr = (gms.gms_batch_sort_item * len(sorts))(*sorts)
+            return emu.gms_batch_plan_mixed(h, None, 0, arr, len(sorts), None)
+        assert plan(gms.gms_batch_sort_item(p(a), None, p(b), None, 64, 3)) == 1   # 3-bit digits
+        assert plan(gms.gms_batch_sort_item(p(a), p(a), p(b), None, 64, 8)) == 1   # value out missing
+        assert plan(gms.gms_batch_sort_item(p(a), None, p(a) + 128, None, 64, 8)) == 7  # in/out overlap
+        assert plan(gms.gms_batch_sort_item(p(a), None, p(b), None, 32, 8),
+                    gms.gms_batch_sort_item(p(a), None, p(b) + 64, None, 32, 8)) == 7  # outputs overlap
+        assert plan(gms.gms_batch_sort_item(p(a), None, p(b), None, 64, 8)) == 0
+    finally:
+        emu.gms_batch_destroy(h)
 
 
 # ---------------------------------------------------------------------------
@@ -292,6 +396,56 @@ def test_emulated_server_budget_admission(emu):
         order = np.argsort(keys, kind="stable")
         np.testing.assert_array_equal(kout, keys[order])
         np.testing.assert_array_equal(vout, vals[order])
+
+
+def test_emulated_server_fragmented_arena_fails_request_instead_of_hanging(emu):
+    """The budget check counts reserved bytes, not contiguous ones: with the
+    arena's free space cut into 4 KiB holes (the emulation build's hold hook
+    plays the sorts' buffers), a request whose step tables need more than a
+    hole is admitted but can never be placed.  With nothing in flight it must
+    fail with GMS_ERR_CAPACITY (not wait forever); once the space is whole
+    again the same request succeeds."""
+    emu.gms_emu_server_hold.restype = ctypes.c_int
+    emu.gms_emu_server_hold.argtypes = [ctypes.c_void_p, ctypes.c_size_t,
+                                        ctypes.POINTER(ctypes.c_size_t)]
+    emu.gms_emu_server_unhold.restype = ctypes.c_int
+    emu.gms_emu_server_unhold.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t]
+    h = _emu_server(emu, 1 << 20, 2)
+    piece = 4096
+    held = []
+    try:
+        off = ctypes.c_size_t()
+        while emu.gms_emu_server_hold(h, piece, ctypes.byref(off)) == 0:
+            held.append(off.value)
+        assert len(held) == (1 << 20) // piece
+        for o in held[::2]:                              # 4 KiB holes
+            assert emu.gms_emu_server_unhold(h, o, piece) == 0
+        held = held[1::2]
+        n, B = 50000, 256
+        keys = W.uniform_keys(n, seed=404)
+        vals = np.arange(n, dtype=np.uint32)
+        kout = np.zeros(n, np.uint32)
+        vout = np.zeros(n, np.uint32)
+        req = gms.gms_request(0, p(keys), p(vals), p(kout), p(vout), n, B,
+                              gms.range_bucket(B).desc(), 0, None)
+        t = ctypes.c_uint64()
+        assert emu.gms_server_submit(h, ctypes.byref(req), ctypes.byref(t)) == 0
+        assert emu.gms_server_wait(h, t.value, None) == 3        # GMS_ERR_CAPACITY
+        for o in held:
+            assert emu.gms_emu_server_unhold(h, o, piece) == 0
+        held = []
+        assert emu.gms_server_submit(h, ctypes.byref(req), ctypes.byref(t)) == 0
+        assert emu.gms_server_wait(h, t.value, None) == 0
+        wk, wv, _ = oracle.multisplit(keys, B, oracle.RANGE, 0, vals)
+        np.testing.assert_array_equal(kout, wk)
+        np.testing.assert_array_equal(vout, wv)
+        st = gms.gms_server_stats()
+        emu.gms_server_get_stats(h, ctypes.byref(st))
+        assert st.completed == 2 and st.active == 0
+    finally:
+        for o in held:
+            emu.gms_emu_server_unhold(h, o, piece)
+        emu.gms_server_destroy(h)
 
 
 @pytest.mark.parametrize("B,kind,param,width", [(257, oracle.RANGE, 0, 0), (1000, oracle.DELTA, 4294968, 0),

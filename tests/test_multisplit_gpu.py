@@ -55,13 +55,19 @@ def check_equal(got, want, kv):
         np.testing.assert_array_equal(got[2], want[2])
 
 
-TILE_EDGE_SIZES = [0, 1, 2, 63, 64, 65, 1000, 4095, 4096, 4097, 3 * 4096 + 1, 8192, 100_003]
+def tile_edge_sizes(B, kv):
+    """Sizes around the tile boundaries of this bucket count's own tile
+    (gms.tile_keys): empty, tiny, one wave, tile-1 / tile / tile+1, two
+    tiles +-1, and a ragged multi-tile input."""
+    tile = gms.tile_keys(B, kv, 1 << 20) if B > 1 else 6144
+    return sorted({0, 1, 2, 63, 64, 65, 1000, tile - 1, tile, tile + 1, 2 * tile - 1, 2 * tile,
+                   2 * tile + 1, 3 * tile + 17, 100_003})
 
 
 @pytest.mark.parametrize("kv", [False, True])
 @pytest.mark.parametrize("B", [1, 2, 3, 4, 5, 8, 16, 17, 32, 33, 64, 100, 128, 129, 255, 256])
 def test_range_identifier_all_bucket_counts_ragged(dev, B, kv):
-    for n in TILE_EDGE_SIZES:
+    for n in tile_edge_sizes(B, kv):
         banner(f"range B={B} kv={kv} n={n}")
         keys = W.uniform_keys(n, seed=B * 131 + n)
         got = run_ms(dev, keys, B, gms.range_bucket(B), kv)
@@ -265,12 +271,41 @@ def test_error_paths(dev):
 _KEY_CACHE = {}
 
 
+def _device_skewed_keys(dev, n, dist, B, seed):
+    """Keys of the skewed sweep distributions generated on the GPU (2^27 keys
+    in well under a second), for a power-of-two B under the range identifier:
+    bucket ids first, then key = id << (32 - log2 B) | uniform low bits.
+    zipf: ids by inverse CDF of P(b) ~ 1/(b+1)^1.1; onehot90: 90 % of keys in
+    one random bucket, the rest uniform, in random positions."""
+    import torch
+    assert B & (B - 1) == 0 and B >= 2
+    lb = B.bit_length() - 1
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    if dist == "zipf":
+        w = 1.0 / torch.arange(1, B + 1, dtype=torch.float64, device=dev) ** 1.1
+        cdf = torch.cumsum(w / w.sum(), 0)
+        u = torch.rand(n, dtype=torch.float64, device=dev, generator=g)
+        ids = torch.searchsorted(cdf, u, right=True).clamp_(max=B - 1)
+    elif dist == "onehot90":
+        hot = int(torch.randint(0, B, (1,), device=dev, generator=g).item())
+        ids = torch.randint(0, B, (n,), dtype=torch.int64, device=dev, generator=g)
+        ids[torch.rand(n, device=dev, generator=g) < 0.9] = hot
+    else:
+        raise ValueError(dist)
+    low = torch.randint(0, 1 << (32 - lb), (n,), dtype=torch.int64, device=dev, generator=g)
+    k = (ids.to(torch.int64) << (32 - lb)) | low
+    return (k - ((k >> 31) << 32)).to(torch.int32)   # the same 32 bits as int32
+
+
 def _full_size_keys(dev, n, dist, B):
     import torch
     if dist == "uniform":
         g = torch.Generator(device=dev)
         g.manual_seed(n)
         return torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=dev, generator=g)
+    if n > (1 << 25):
+        return _device_skewed_keys(dev, n, dist, B, seed=n + B)
     key = (n, dist, B)
     if key not in _KEY_CACHE:
         _KEY_CACHE.clear()
@@ -348,6 +383,16 @@ def test_plain_sweep_2e25(dev, dist, B, kv):
 @pytest.mark.parametrize("kv", [False, True])
 def test_plain_sweep_2e27_uniform(dev, B, kv):
     _check_properties(dev, 1 << 27, "uniform", B, kv)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("dist", ["zipf", "onehot90"])
+@pytest.mark.parametrize("B", [2, 8, 32, 64, 256])
+def test_plain_sweep_2e27_skewed(dev, dist, B):
+    """The plain-multisplit sweep at 2^27 keys with Zipf s=1.1 and one-hot
+    90 % bucket ids, key-only and key-value, by the complete properties."""
+    for kv in (False, True):
+        _check_properties(dev, 1 << 27, dist, B, kv)
 
 
 @pytest.mark.slow

@@ -341,3 +341,31 @@ def test_oracle_sort_wide_against_numpy(digit_bits, vbits):
     np.testing.assert_array_equal(ko, keys[order])
     if vbits:
         np.testing.assert_array_equal(vo, vals[order])
+
+
+def test_standard_library_baselines_against_numpy():
+    """The CPU baselines bench.py times (oracle/baseline.cpp) compute what the
+    rows compute: a stable sort equals numpy's stable argsort order, the
+    counting loop equals bincount, the stable partition equals boolean
+    selection in input order."""
+    g = np.random.default_rng(5)
+    n = 50_003
+    keys = g.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+    keys[::7] = keys[3]                                    # ties
+    vals = np.arange(n, dtype=np.uint32)
+    ko, vo = oracle.baseline_stable_sort(keys, vals)
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(ko, keys[order])
+    np.testing.assert_array_equal(vo, vals[order])
+    ko, vo = oracle.baseline_stable_sort(keys)
+    np.testing.assert_array_equal(ko, np.sort(keys))
+    assert vo is None
+    for shift in (0, 8, 24):
+        np.testing.assert_array_equal(oracle.baseline_histogram(keys, 256, shift),
+                                      np.bincount((keys >> np.uint32(shift)) & 0xFF, minlength=256))
+    for thr in (0, 1 << 31, int(0.05 * (1 << 32)), 1 << 32):
+        keep = keys.astype(np.uint64) < thr
+        c, ko, vo = oracle.baseline_stable_partition(keys, thr, vals)
+        assert c == int(keep.sum())
+        np.testing.assert_array_equal(ko, np.concatenate([keys[keep], keys[~keep]]))
+        np.testing.assert_array_equal(vo, np.concatenate([vals[keep], vals[~keep]]))

@@ -318,3 +318,59 @@ def test_compaction_like_reference_driver_and_device_count(dev):
         assert int(d_cnt.item()) == c2
         np.testing.assert_array_equal(from_dev(kout2)[:n], wk)
         gms.compaction_release_memory(ctx)
+
+
+@pytest.mark.slow
+def test_histogram_256_low_byte_2e27(dev):
+    """The histogram cell of the sweep at its stated size: 256 bins of the
+    low 8 bits of 2^27 uniform keys, against the oracle's counting loop."""
+    import torch
+    n = 1 << 27
+    banner(f"histogram 256 bins n={n}")
+    g = torch.Generator(device=dev)
+    g.manual_seed(227)
+    kin = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=dev, generator=g)
+    h = torch.zeros(256, dtype=torch.int32, device=dev)
+    gms.histogram(kin, n, h, 256, gms.ms_bucket_sort_8bit(0))
+    keys = from_dev(kin)
+    np.testing.assert_array_equal(from_dev(h).astype(np.int64),
+                                  oracle.histogram(keys, 256, oracle.DIGIT, 0).astype(np.int64))
+    assert int(from_dev(h).astype(np.int64).sum()) == n
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("selectivity", [0.5, 0.05])
+def test_compaction_2e27(dev, selectivity):
+    """2-bucket compaction of 2^27 keys at the sweep's selectivities, key-only
+    and key-value: the output is the stable partition (kept keys first, in
+    input order, then the rest in input order) and the count is the
+    partition point."""
+    import torch
+    n = 1 << 27
+    thr = int(selectivity * (1 << 32))
+    g = torch.Generator(device=dev)
+    g.manual_seed(27 + int(selectivity * 100))
+    kin = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device=dev, generator=g)
+    keep = (kin.to(torch.int64) & 0xFFFFFFFF) < thr
+    order = torch.cat([torch.nonzero(keep).flatten(), torch.nonzero(~keep).flatten()])
+    want_cnt = int(keep.sum().item())
+    del keep
+    for kv in (False, True):
+        banner(f"compaction n={n} selectivity={selectivity} kv={kv}")
+        ctx = gms.compaction_context()
+        (gms.compaction_allocate_key_value if kv else gms.compaction_allocate_key_only)(n, ctx)
+        kout = torch.empty_like(kin)
+        if kv:
+            vin = torch.arange(n, dtype=torch.int32, device=dev)
+            vout = torch.empty_like(vin)
+            cnt = gms.compaction_key_value(kin, vin, kout, vout, n, ctx, gms.delta_func(thr))
+        else:
+            cnt = gms.compaction_key_only(kin, kout, n, ctx, gms.delta_func(thr))
+        torch.cuda.synchronize()
+        gms.compaction_release_memory(ctx)
+        assert cnt == want_cnt
+        assert bool((kout == kin[order]).all())
+        if kv:
+            assert bool((vout.to(torch.int64) == order).all())
+            del vin, vout
+        del kout
