@@ -122,6 +122,7 @@ _SIGS = [
     ("gms_multisplit_key_only", _I, [_P, _P, _P, _U32, gms_bucket_desc, _P, _P]),
     ("gms_multisplit_key_value", _I, [_P, _P, _P, _P, _P, _U32, gms_bucket_desc, _P, _P]),
     ("gms_multisplit", _I, [_P, _P, _P, _P, _P, _U32, gms_bucket_desc, _P, _P]),
+    ("gms_multisplit_noalloc", _I, [_P, _P, _P, _P, _U32, _U32, gms_bucket_desc, _P, _P]),
     ("gms_multisplit_wide", _I, [_P, _P, _P, _P, _P, _U32, _U32, _U32, gms_bucket_desc64, _P, _P,
                                  _P]),
     ("gms_multisplit_set_variant", _I, [_P, _I]),
@@ -542,6 +543,20 @@ def multisplit(d_key_in, d_key_out, num_elements: int, context: multisplit_conte
     _check(lib().gms_multisplit(context.handle, _ptr(d_key_in), _ptr(d_value_in), _ptr(d_key_out),
                                 _ptr(d_value_out), int(num_elements), bucket_identifier.desc(),
                                 _ptr(d_offsets), st), "gms_multisplit")
+
+
+def multisplit_noalloc(d_key_in, d_key_out, num_elements: int, num_buckets: int,
+                       bucket_identifier, d_value_in=None, d_value_out=None, d_offsets=None,
+                       stream=None) -> None:
+    """:func:`multisplit` without a context: allocates nothing; scratch is the
+    library's per-GPU array of GMS_NOALLOC_SCRATCH_BYTES (32 MiB), shared by
+    every call on that GPU, so calls that overlap in time on one GPU race.
+    Up to 1024 buckets, 32-bit keys and values."""
+    st = stream if stream is not None else _stream(d_key_in, d_key_out)
+    _check(lib().gms_multisplit_noalloc(_ptr(d_key_in), _ptr(d_value_in), _ptr(d_key_out),
+                                        _ptr(d_value_out), int(num_elements), int(num_buckets),
+                                        bucket_identifier.desc(), _ptr(d_offsets), st),
+           "gms_multisplit_noalloc")
 
 
 def set_profiling(context: multisplit_context, enable: bool) -> None:

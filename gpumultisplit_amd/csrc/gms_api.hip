@@ -3,6 +3,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 
 #include "gms_internal.hpp"
@@ -381,6 +382,36 @@ static gms_status stage_plan(Scratch& s, uint32_t capacity, int allocated, uint3
 
 }  // namespace gms
 
+// ---------------------------------------------------------------- no-alloc form
+// Scratch of gms_multisplit_noalloc: the per-tile count matrix, then the scan
+// partials.  One copy per GPU, held until the process ends.  In host code
+// `scratch` names a host-side stand-in, never the device memory: host code
+// uses only the address hipGetSymbolAddress returns on each device.
+constexpr size_t kNoAllocWords = GMS_NOALLOC_SCRATCH_BYTES / sizeof(uint32_t);
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+namespace {
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu;
+uint32_t* g_noalloc_ptr[kNoAllocDevices] = {};
+}  // namespace
+
+// The current device's scratch address, looked up on the first call there.
+static gms_status noalloc_scratch(uint32_t** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices)
+        return GMS_ERR_NO_DEVICE;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        uint32_t* ptr = nullptr;
+        if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+            return GMS_ERR_HIP;
+        g_noalloc_ptr[dev] = ptr;
+    }
+    *out = g_noalloc_ptr[dev];
+    return GMS_SUCCESS;
+}
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -493,13 +524,10 @@ gms_status gms_multisplit_destroy(gms_multisplit_ctx* ctx) {
     return GMS_SUCCESS;
 }
 
-static gms_status validate_call(gms_multisplit_ctx* ctx, const uint32_t* kin, const uint32_t* vin,
-                                uint32_t* kout, uint32_t* vout, uint32_t n, gms_bucket_desc d) {
-    if (!ctx) return GMS_ERR_INVALID_ARG;
-    if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
-    if (n > ctx->capacity) return GMS_ERR_CAPACITY;
-    gms_status st = check_desc(ctx->num_buckets, d);
-    if (st != GMS_SUCCESS) return st;
+// Pointers of one call of n 32-bit elements: present, values both or neither,
+// no output overlapping an input or the other output.
+static gms_status check_buffers(const uint32_t* kin, const uint32_t* vin, const uint32_t* kout,
+                                const uint32_t* vout, uint32_t n) {
     if (n == 0) return GMS_SUCCESS;
     if (!kin || !kout) return GMS_ERR_INVALID_ARG;
     if ((vin == nullptr) != (vout == nullptr)) return GMS_ERR_INVALID_ARG;
@@ -511,6 +539,16 @@ static gms_status validate_call(gms_multisplit_ctx* ctx, const uint32_t* kin, co
             return GMS_ERR_ALIAS;
     }
     return GMS_SUCCESS;
+}
+
+static gms_status validate_call(gms_multisplit_ctx* ctx, const uint32_t* kin, const uint32_t* vin,
+                                uint32_t* kout, uint32_t* vout, uint32_t n, gms_bucket_desc d) {
+    if (!ctx) return GMS_ERR_INVALID_ARG;
+    if (!ctx->allocated) return GMS_ERR_NOT_ALLOCATED;
+    if (n > ctx->capacity) return GMS_ERR_CAPACITY;
+    gms_status st = check_desc(ctx->num_buckets, d);
+    if (st != GMS_SUCCESS) return st;
+    return check_buffers(kin, vin, kout, vout, n);
 }
 
 // The wave-level variant where it was chosen for the context (or where the
@@ -558,6 +596,36 @@ static gms_status multisplit_host_offsets(gms_multisplit_ctx* ctx, const uint32_
                        hipMemcpyDeviceToHost, s) != hipSuccess)
         return GMS_ERR_HIP;
     return hip_status(hipStreamSynchronize(s));
+}
+
+gms_status gms_multisplit_noalloc(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                                  uint32_t* vout, uint32_t n, uint32_t num_buckets,
+                                  gms_bucket_desc d, uint32_t* d_offsets, void* stream) {
+    if (num_buckets < 1 || num_buckets > kSinglePassBuckets) return GMS_ERR_UNSUPPORTED_BUCKETS;
+    gms_status st = check_desc(num_buckets, d);
+    if (st != GMS_SUCCESS) return st;
+    // The matrix, padded to 256 bytes, then the partials of its scan.  Every
+    // matrix word and partial is written before it is read, so nothing is
+    // cleared between calls.
+    const bool kv = vin != nullptr;
+    const int lb = ceil_log2_u32(num_buckets);
+    const uint32_t tile = tile_keys(lb, kv, n);
+    const size_t m = ((size_t)1 << lb) * (((size_t)n + tile - 1) / tile);
+    const size_t m_pad = (m + 63) / 64 * 64;
+    if (m_pad > kNoAllocWords || scan_partials_needed(m) > kNoAllocWords - m_pad)
+        return GMS_ERR_CAPACITY;
+    st = check_buffers(kin, vin, kout, vout, n);
+    if (st != GMS_SUCCESS) return st;
+    uint32_t* base = nullptr;
+    st = noalloc_scratch(&base);
+    if (st != GMS_SUCCESS) return st;
+    Scratch s;
+    s.hist = base;
+    s.hist_cap = m;
+    s.part = base + m_pad;
+    s.part_cap = kNoAllocWords - m_pad;
+    return run_multisplit(s, kin, vin, kout, vout, n, num_buckets, d, d_offsets,
+                          (hipStream_t)stream, nullptr);
 }
 
 gms_status gms_multisplit_key_only(gms_multisplit_ctx* ctx, const uint32_t* kin, uint32_t* kout,

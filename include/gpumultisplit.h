@@ -142,6 +142,27 @@ gms_status gms_multisplit(gms_multisplit_ctx *ctx, const uint32_t *d_key_in,
                           uint32_t *d_value_out, uint32_t n, gms_bucket_desc desc,
                           uint32_t *d_offsets, void *stream);
 
+/* Size of the device scratch of gms_multisplit_noalloc, one copy per GPU. */
+#define GMS_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* No counterpart in the reference.  gms_multisplit without a context, for
+ * callers that cannot allocate (code under hipGraph capture, serving loops,
+ * harnesses that forbid hipMalloc): it allocates nothing, on the device or
+ * the host.  Its scratch (the per-tile count matrix and its scan partials) is
+ * a file-scope device array of GMS_NOALLOC_SCRATCH_BYTES in the library, one
+ * copy on each GPU, held until the process ends.  Up to 1024 buckets; the
+ * call needs 2^ceil(log2 B) * ceil(n / gms_tile_keys(B, key_value, n)) words
+ * of matrix plus about 1/256 of that for the scan, and returns
+ * GMS_ERR_CAPACITY beyond the array (2^27 key-value elements in 256 buckets
+ * take 22 MiB).  Same results as gms_multisplit.  Every call on one GPU
+ * shares that GPU's array: calls that overlap in time on one GPU (two
+ * streams, two host threads) race and give wrong results; calls on one
+ * stream are ordered and safe. */
+gms_status gms_multisplit_noalloc(const uint32_t *d_key_in, const uint32_t *d_value_in,
+                                  uint32_t *d_key_out, uint32_t *d_value_out, uint32_t n,
+                                  uint32_t num_buckets, gms_bucket_desc desc,
+                                  uint32_t *d_offsets, void *stream);
+
 /* Multisplit of 32- or 64-bit keys carrying 32- or 64-bit values: the
  * reference's multisplit_key_only<KeyT> / multisplit_key_value<KeyT, ValueT>
  * (src/api/multisplit.cuh:81-197, :199-313) for KeyT, ValueT of 4 or 8 bytes.

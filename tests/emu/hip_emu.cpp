@@ -102,4 +102,8 @@ void launch(dim3 grid, dim3 block, std::function<void()> body) {
     }
 }
 
+std::atomic<uint64_t> alloc_calls{0};
 }  // namespace gms_emu
+
+// Allocation calls (hipMalloc, hipFree, hipHostMalloc, hipHostFree) so far.
+extern "C" uint64_t gms_emu_alloc_calls() { return gms_emu::alloc_calls.load(); }

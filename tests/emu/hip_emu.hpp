@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <barrier>
 #include <chrono>
 #include <cstdio>
@@ -27,6 +28,7 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) __attribute__((aligned(n)))
 
 struct gms_emu_dim3 {
     uint32_t x, y, z;
@@ -117,7 +119,18 @@ inline hipError_t hipGetDevice(int* d) {
     *d = 0;
     return hipSuccess;
 }
+namespace gms_emu {
+// Device and pinned allocations and frees made so far (gms_emu_alloc_calls).
+extern std::atomic<uint64_t> alloc_calls;
+}
+// A __device__ variable is a host global here: its address is the device one.
+#define HIP_SYMBOL(x) reinterpret_cast<const void*>(&(x))
+inline hipError_t hipGetSymbolAddress(void** p, const void* symbol) {
+    *p = const_cast<void*>(symbol);
+    return hipSuccess;
+}
 inline hipError_t hipMalloc(void** p, size_t bytes) {
+    ++gms_emu::alloc_calls;
     *p = aligned_alloc(256, ((bytes + 255) / 256) * 256 + 256);
     return *p ? hipSuccess : hipErrorOutOfMemory;
 }
@@ -126,6 +139,7 @@ inline hipError_t hipMalloc(T** p, size_t bytes) {
     return hipMalloc((void**)p, bytes);
 }
 inline hipError_t hipFree(void* p) {
+    ++gms_emu::alloc_calls;
     free(p);
     return hipSuccess;
 }
@@ -166,10 +180,12 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
 // Pinned host memory is ordinary host memory here.
 #define hipHostMallocDefault 0u
 inline hipError_t hipHostMalloc(void** p, size_t bytes, unsigned) {
+    ++gms_emu::alloc_calls;
     *p = aligned_alloc(256, ((bytes + 255) / 256) * 256);
     return *p ? hipSuccess : hipErrorOutOfMemory;
 }
 inline hipError_t hipHostFree(void* p) {
+    ++gms_emu::alloc_calls;
     free(p);
     return hipSuccess;
 }

@@ -722,3 +722,56 @@ def test_emulated_compaction_wide(emu, kbytes, vbytes):
     assert cnt.value == wo[1] == int(np.sum((keys & 1) == 0))
     if vbytes:
         np.testing.assert_array_equal(vout, wv)
+
+
+# ---------------------------------------------------------------------------
+# gms_multisplit_noalloc on the emulation: equal to gms_multisplit on the same
+# input, on two calls in a row (the second smaller, so stale words of the
+# first call's scratch sit past the second call's matrix), with no allocation
+# or free while it runs; over-capacity inputs rejected before any launch.
+# ---------------------------------------------------------------------------
+def test_emulated_noalloc_equals_context_call(emu):
+    emu.gms_emu_alloc_calls.restype = ctypes.c_uint64
+    for B, kv, sizes in ((256, True, (70_001, 9000)), (5, False, (20_000, 3)),
+                         (1024, True, (30_000, 6145))):
+        h = ctypes.c_void_p()
+        assert emu.gms_multisplit_create(B, ctypes.byref(h)) == 0
+        assert emu.gms_multisplit_allocate(h, max(sizes), int(kv)) == 0
+        f = gms.range_bucket(B).desc()
+        try:
+            for n in sizes:
+                keys = W.keys_for("zipf", n, B, seed=n) if B >= 8 else W.uniform_keys(n, seed=n)
+                vals = np.arange(n, dtype=np.uint32) if kv else None
+                outs = []
+                for noalloc in (False, True):
+                    kout = np.full(n, 0x5A5A5A5A, np.uint32)
+                    vout = np.full(n, 0x5A5A5A5A, np.uint32) if kv else None
+                    offs = np.full(B, 0xFFFFFFFF, np.uint32)
+                    before = emu.gms_emu_alloc_calls()
+                    if noalloc:
+                        st = emu.gms_multisplit_noalloc(p(keys), p(vals), p(kout), p(vout), n, B, f,
+                                                        p(offs), None)
+                        assert emu.gms_emu_alloc_calls() == before, "no-alloc call allocated"
+                    else:
+                        st = emu.gms_multisplit(h, p(keys), p(vals), p(kout), p(vout), n, f,
+                                                p(offs), None)
+                    assert st == 0
+                    outs.append((kout, vout, offs))
+                (k0, v0, o0), (k1, v1, o1) = outs
+                np.testing.assert_array_equal(k1, k0, err_msg=f"B={B} n={n} keys")
+                np.testing.assert_array_equal(o1, o0, err_msg=f"B={B} n={n} offsets")
+                if kv:
+                    np.testing.assert_array_equal(v1, v0, err_msg=f"B={B} n={n} values")
+                wk, _, wo = oracle.multisplit(keys, B, oracle.RANGE, 0, vals)
+                np.testing.assert_array_equal(k1, wk)
+                np.testing.assert_array_equal(o1, wo)
+        finally:
+            emu.gms_multisplit_destroy(h)
+    # beyond the 32 MiB array: 2^31 keys in 1024 buckets; over 1024 buckets
+    a = np.zeros(4, np.uint32)
+    b = np.zeros(4, np.uint32)
+    big = (1 << 31)
+    assert emu.gms_multisplit_noalloc(p(a), None, p(b), None, big, 1024,
+                                      gms.range_bucket(1024).desc(), None, None) == 3
+    assert emu.gms_multisplit_noalloc(p(a), None, p(b), None, 4, 1025,
+                                      gms.range_bucket(1025).desc(), None, None) == 2
