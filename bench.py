@@ -461,10 +461,11 @@ def cpu_baseline_aux(w, n, reps):
 
 
 def bench_variants(args, torch, gms, dev):
-    """Block-wide (BMS) against wave-level (WMS) multisplit: one JSON line per
-    (buckets, key-only/key-value, n) with both variants' time per call, the
-    measurement the automatic variant choice is set from.  Uniform keys,
-    range identifier, values = index; --steps timed calls after --warmup."""
+    """Block-wide (BMS) against wave-level (WMS) and direct wave-level (DMS)
+    multisplit: one JSON line per (buckets, key-only/key-value, n) with each
+    variant's time per call, the measurement the automatic variant choice is
+    set from.  Uniform keys, range identifier, values = index; --steps timed
+    calls after --warmup."""
     L = gms.lib()
     stream = torch.cuda.current_stream(dev).cuda_stream
     g = torch.Generator(device=dev)
@@ -481,7 +482,8 @@ def bench_variants(args, torch, gms, dev):
             for log2n in (16, 20, 24, 27):
                 n = 1 << log2n
                 res = {}
-                for name, var in (("bms", gms.VARIANT_BMS), ("wms", gms.VARIANT_WMS)):
+                for name, var in (("bms", gms.VARIANT_BMS), ("wms", gms.VARIANT_WMS),
+                                  ("dms", gms.VARIANT_DMS)):
                     gms.set_variant(ctx, var)
 
                     def step():
@@ -503,7 +505,9 @@ def bench_variants(args, torch, gms, dev):
                     res[name] = e0.elapsed_time(e1) / args.steps
                 print(json.dumps({"buckets": B, "key_value": kv, "n": n,
                                   "bms_ms": round(res["bms"], 5), "wms_ms": round(res["wms"], 5),
-                                  "wms_over_bms": round(res["wms"] / res["bms"], 3)}), flush=True)
+                                  "dms_ms": round(res["dms"], 5),
+                                  "wms_over_bms": round(res["wms"] / res["bms"], 3),
+                                  "dms_over_bms": round(res["dms"] / res["bms"], 3)}), flush=True)
         gms.set_variant(ctx, gms.VARIANT_AUTO)
         gms.multisplit_release_memory(ctx)
 

@@ -37,7 +37,7 @@ STATUS_NAMES = {
 }
 
 BUCKET_RANGE, BUCKET_DELTA, BUCKET_DIGIT, BUCKET_IDENTITY, BUCKET_LOWBITS = 0, 1, 2, 3, 4
-VARIANT_AUTO, VARIANT_BMS, VARIANT_WMS = 0, 1, 2
+VARIANT_AUTO, VARIANT_BMS, VARIANT_WMS, VARIANT_DMS = 0, 1, 2, 3
 
 
 class GmsError(RuntimeError):
@@ -75,7 +75,16 @@ class gms_batch_item(ctypes.Structure):
 class gms_batch_sort_item(ctypes.Structure):
     _fields_ = [("d_key_in", ctypes.c_void_p), ("d_value_in", ctypes.c_void_p),
                 ("d_key_out", ctypes.c_void_p), ("d_value_out", ctypes.c_void_p),
-                ("n", ctypes.c_uint32), ("digit_bits", ctypes.c_uint32)]
+                ("n", ctypes.c_uint32), ("digit_bits", ctypes.c_uint32),
+                ("key_bytes", ctypes.c_uint32), ("value_bytes", ctypes.c_uint32)]
+
+
+class gms_batch_wide_item(ctypes.Structure):
+    _fields_ = [("d_key_in", ctypes.c_void_p), ("d_value_in", ctypes.c_void_p),
+                ("d_key_out", ctypes.c_void_p), ("d_value_out", ctypes.c_void_p),
+                ("n", ctypes.c_uint32), ("num_buckets", ctypes.c_uint32),
+                ("key_bytes", ctypes.c_uint32), ("value_bytes", ctypes.c_uint32),
+                ("desc", gms_bucket_desc64), ("d_offsets", ctypes.c_void_p)]
 
 
 class gms_batch_tuning(ctypes.Structure):
@@ -87,7 +96,9 @@ class gms_request(ctypes.Structure):
                 ("d_value_in", ctypes.c_void_p), ("d_key_out", ctypes.c_void_p),
                 ("d_value_out", ctypes.c_void_p), ("n", ctypes.c_uint32),
                 ("num_buckets", ctypes.c_uint32), ("desc", gms_bucket_desc),
-                ("digit_bits", ctypes.c_uint32), ("d_offsets", ctypes.c_void_p)]
+                ("digit_bits", ctypes.c_uint32), ("d_offsets", ctypes.c_void_p),
+                ("key_bytes", ctypes.c_uint32), ("value_bytes", ctypes.c_uint32),
+                ("desc64", gms_bucket_desc64)]
 
 
 class gms_server_config(ctypes.Structure):
@@ -166,6 +177,9 @@ _SIGS = [
     ("gms_batch_plan", _I, [_P, ctypes.POINTER(gms_batch_item), _U32, _P]),
     ("gms_batch_plan_mixed", _I, [_P, ctypes.POINTER(gms_batch_item), _U32,
                                   ctypes.POINTER(gms_batch_sort_item), _U32, _P]),
+    ("gms_batch_plan_wide", _I, [_P, ctypes.POINTER(gms_batch_item), _U32,
+                                 ctypes.POINTER(gms_batch_wide_item), _U32,
+                                 ctypes.POINTER(gms_batch_sort_item), _U32, _P]),
     ("gms_batch_run", _I, [_P, _P]),
     ("gms_batch_capture", _I, [_P]),
     ("gms_batch_launch_graph", _I, [_P, _P]),
@@ -404,8 +418,10 @@ def subproblem_size_wms_key_value(num_buckets: int):
 
 
 def set_variant(context, variant: int) -> None:
-    """Block-wide (VARIANT_BMS), wave-level (VARIANT_WMS, up to 64 buckets)
-    or automatic (VARIANT_AUTO) kernels for the context's multisplits."""
+    """Block-wide (VARIANT_BMS), wave-level (VARIANT_WMS, up to 64 buckets),
+    wave-level with direct scatter (VARIANT_DMS, up to 64 buckets) or
+    automatic (VARIANT_AUTO, which is BMS) kernels for the context's
+    multisplits."""
     _check(lib().gms_multisplit_set_variant(context.handle, int(variant)), "set_variant")
 
 
@@ -825,8 +841,9 @@ class BatchItem:
 class BatchSortItem:
     """One stable radix sort of a batch: the digit schedule of
     ``multisplit_sort_<digit_bits>bit[_pairs]`` (src/api/multisplit_sort.cuh:
-    320-559), as :func:`sort`.  The outputs double as a pass buffer; the
-    input is never written."""
+    320-559), as :func:`sort`; 32- or 64-bit keys and values, from the
+    tensors' element size.  The outputs double as a pass buffer; the input is
+    never written."""
 
     def __init__(self, d_key_in, d_key_out, num_elements: int, digit_bits: int = 8,
                  d_value_in=None, d_value_out=None):
@@ -834,13 +851,44 @@ class BatchSortItem:
         self.d_value_in, self.d_value_out = d_value_in, d_value_out
         self.n = int(num_elements)
         self.digit_bits = int(digit_bits)
+        self.key_bytes = _elem_bytes(d_key_in, d_key_out)
+        self.value_bytes = (_elem_bytes(d_value_in, d_value_out)
+                            if d_value_in is not None or d_value_out is not None else 4)
         for name in ("d_key_in", "d_key_out", "d_value_in", "d_value_out"):
             _check_len(name, getattr(self, name), self.n)
 
     def c_item(self) -> gms_batch_sort_item:
-        return gms_batch_sort_item(_ptr(self.d_key_in), _ptr(self.d_value_in),
-                                   _ptr(self.d_key_out), _ptr(self.d_value_out), self.n,
-                                   self.digit_bits)
+        return gms_batch_sort_item(_ptr(self.d_key_in, (4, 8)), _ptr(self.d_value_in, (4, 8)),
+                                   _ptr(self.d_key_out, (4, 8)), _ptr(self.d_value_out, (4, 8)),
+                                   self.n, self.digit_bits, self.key_bytes, self.value_bytes)
+
+
+class BatchWideItem:
+    """One multisplit of a batch whose keys and/or values are 64-bit (int64
+    tensors), as :func:`multisplit` on such tensors (the reference's
+    KeyT/ValueT templates, src/api/multisplit.cuh:81-313); up to 256
+    buckets, identifiers with their 64-bit parameter."""
+
+    def __init__(self, d_key_in, d_key_out, num_elements: int, num_buckets: int,
+                 bucket_identifier=None, d_value_in=None, d_value_out=None, d_offsets=None):
+        self.d_key_in, self.d_key_out = d_key_in, d_key_out
+        self.d_value_in, self.d_value_out = d_value_in, d_value_out
+        self.n = int(num_elements)
+        self.num_buckets = int(num_buckets)
+        self.f = bucket_identifier if bucket_identifier is not None else range_bucket(num_buckets)
+        self.d_offsets = d_offsets
+        self.key_bytes = _elem_bytes(d_key_in, d_key_out)
+        self.value_bytes = (_elem_bytes(d_value_in, d_value_out)
+                            if d_value_in is not None or d_value_out is not None else 0)
+        for name in ("d_key_in", "d_key_out", "d_value_in", "d_value_out"):
+            _check_len(name, getattr(self, name), self.n)
+        _check_len("d_offsets", d_offsets, self.num_buckets)
+
+    def c_item(self) -> gms_batch_wide_item:
+        return gms_batch_wide_item(_ptr(self.d_key_in, (4, 8)), _ptr(self.d_value_in, (4, 8)),
+                                   _ptr(self.d_key_out, (4, 8)), _ptr(self.d_value_out, (4, 8)),
+                                   self.n, self.num_buckets, self.key_bytes, self.value_bytes,
+                                   self.f.desc64(), _ptr(self.d_offsets))
 
 
 class Batch:
@@ -853,19 +901,22 @@ class Batch:
     of every sort in step 0, the p-th pass of every sort in step p.
     """
 
-    def __init__(self, items, stream=None, sorts=()):
+    def __init__(self, items, stream=None, sorts=(), wide=()):
         h = ctypes.c_void_p()
         _check(lib().gms_batch_create(ctypes.byref(h)), "gms_batch_create")
         self._h = h
         self.items = list(items)
         self.sorts = list(sorts)
+        self.wide = list(wide)
         arr = (gms_batch_item * max(len(self.items), 1))(*[it.c_item() for it in self.items])
         self._stream = stream
-        if self.sorts:
-            sarr = (gms_batch_sort_item * len(self.sorts))(*[s.c_item() for s in self.sorts])
-            _check(lib().gms_batch_plan_mixed(self._h, arr, len(self.items), sarr,
-                                              len(self.sorts), self._s()),
-                   "gms_batch_plan_mixed")
+        if self.sorts or self.wide:
+            sarr = (gms_batch_sort_item * max(len(self.sorts), 1))(
+                *[s.c_item() for s in self.sorts])
+            warr = (gms_batch_wide_item * max(len(self.wide), 1))(*[w.c_item() for w in self.wide])
+            _check(lib().gms_batch_plan_wide(self._h, arr, len(self.items), warr, len(self.wide),
+                                             sarr, len(self.sorts), self._s()),
+                   "gms_batch_plan_wide")
         else:
             _check(lib().gms_batch_plan(self._h, arr, len(self.items), self._s()),
                    "gms_batch_plan")
@@ -873,7 +924,7 @@ class Batch:
     def _s(self):
         if self._stream is not None:
             return self._stream
-        for it in self.items + self.sorts:
+        for it in self.items + self.sorts + self.wide:
             s = _stream(it.d_key_in, it.d_key_out)
             if s is not None:
                 return s
@@ -930,27 +981,42 @@ class Batch:
 OP_MULTISPLIT, OP_SORT = 0, 1
 
 
+def _request_widths(d_key_in, d_key_out, d_value_in, d_value_out):
+    kb = _elem_bytes(d_key_in, d_key_out)
+    vb = (_elem_bytes(d_value_in, d_value_out)
+          if d_value_in is not None or d_value_out is not None else 4)
+    return kb, vb
+
+
 def multisplit_request(d_key_in, d_key_out, num_elements: int, num_buckets: int,
                        bucket_identifier=None, d_value_in=None, d_value_out=None,
                        d_offsets=None) -> gms_request:
+    """A multisplit request; int64 keys or values make it a 64-bit one (the
+    identifier's parameter may then need 64 bits)."""
     f = bucket_identifier if bucket_identifier is not None else range_bucket(num_buckets)
     for name, t in (("d_key_in", d_key_in), ("d_key_out", d_key_out),
                     ("d_value_in", d_value_in), ("d_value_out", d_value_out)):
         _check_len(name, t, num_elements)
     _check_len("d_offsets", d_offsets, num_buckets)
-    return gms_request(OP_MULTISPLIT, _ptr(d_key_in), _ptr(d_value_in), _ptr(d_key_out),
-                       _ptr(d_value_out), int(num_elements), int(num_buckets), f.desc(), 0,
-                       _ptr(d_offsets))
+    kb, vb = _request_widths(d_key_in, d_key_out, d_value_in, d_value_out)
+    wide = kb == 8 or (vb == 8 and d_value_in is not None)
+    desc = gms_bucket_desc(0, 0, 0) if wide else f.desc()
+    return gms_request(OP_MULTISPLIT, _ptr(d_key_in, (4, 8)), _ptr(d_value_in, (4, 8)),
+                       _ptr(d_key_out, (4, 8)), _ptr(d_value_out, (4, 8)), int(num_elements),
+                       int(num_buckets), desc, 0, _ptr(d_offsets), kb, vb, f.desc64())
 
 
 def sort_request(d_key_in, d_key_out, num_elements: int, digit_bits: int = 8, d_value_in=None,
                  d_value_out=None) -> gms_request:
+    """A radix sort request; int64 keys or values make it a 64-bit one."""
     for name, t in (("d_key_in", d_key_in), ("d_key_out", d_key_out),
                     ("d_value_in", d_value_in), ("d_value_out", d_value_out)):
         _check_len(name, t, num_elements)
-    return gms_request(OP_SORT, _ptr(d_key_in), _ptr(d_value_in), _ptr(d_key_out),
-                       _ptr(d_value_out), int(num_elements), 0, gms_bucket_desc(0, 0, 0),
-                       int(digit_bits), None)
+    kb, vb = _request_widths(d_key_in, d_key_out, d_value_in, d_value_out)
+    return gms_request(OP_SORT, _ptr(d_key_in, (4, 8)), _ptr(d_value_in, (4, 8)),
+                       _ptr(d_key_out, (4, 8)), _ptr(d_value_out, (4, 8)), int(num_elements), 0,
+                       gms_bucket_desc(0, 0, 0), int(digit_bits), None, kb, vb,
+                       gms_bucket_desc64(0, 0, 0))
 
 
 class Server:

@@ -172,7 +172,7 @@ gms_status check_desc(uint32_t num_buckets, gms_bucket_desc d) {
     }
 }
 
-static gms_status check_desc64(uint32_t B, const gms_bucket_desc64& d, uint32_t key_bits) {
+gms_status check_desc64(uint32_t B, const gms_bucket_desc64& d, uint32_t key_bits) {
     switch (d.kind) {
         case GMS_BUCKET_RANGE:
         case GMS_BUCKET_IDENTITY: return GMS_SUCCESS;
@@ -551,22 +551,21 @@ static gms_status validate_call(gms_multisplit_ctx* ctx, const uint32_t* kin, co
     return check_buffers(kin, vin, kout, vout, n);
 }
 
-// The wave-level variant where it was chosen for the context (or where the
-// automatic choice takes it), else the block-wide one.
-static bool use_wms(const gms_multisplit_ctx* ctx, uint32_t n, bool kv) {
+// The wave-level kernels only where the context was set to them: AUTO takes
+// the block-wide ones, which measured as fast or faster at every bucket count
+// and size on MI355X (the wave-level ones at most 5 % faster, at 2 buckets
+// and up to 2^20 keys; 1.6-1.9x slower at 64 buckets and 2^24-2^27 keys).
+static bool use_wms(const gms_multisplit_ctx* ctx) {
     if (ctx->num_buckets < 2 || ctx->num_buckets > kWmsMaxBuckets) return false;
-    if (ctx->variant == GMS_VARIANT_WMS) return true;
-    if (ctx->variant == GMS_VARIANT_BMS) return false;
-    (void)n;
-    (void)kv;
-    return false;
+    return ctx->variant == GMS_VARIANT_WMS || ctx->variant == GMS_VARIANT_DMS;
 }
 
 static gms_status run_ctx(gms_multisplit_ctx* ctx, const uint32_t* kin, const uint32_t* vin,
                           uint32_t* kout, uint32_t* vout, uint32_t n, gms_bucket_desc d,
                           uint32_t* d_offsets, hipStream_t s) {
-    if (use_wms(ctx, n, vin != nullptr))
-        return run_wms_multisplit(ctx->s, kin, vin, kout, vout, n, ctx->num_buckets, d, d_offsets, s);
+    if (use_wms(ctx))
+        return run_wms_multisplit(ctx->s, kin, vin, kout, vout, n, ctx->num_buckets, d, d_offsets,
+                                  ctx->variant == GMS_VARIANT_DMS, s);
     if (ctx->num_buckets > kSinglePassBuckets)
         return run_large_multisplit(ctx->large, ctx->s, kin, vin, kout, vout, n, ctx->num_buckets,
                                     d, d_offsets, s);
@@ -720,9 +719,11 @@ gms_status gms_multisplit_wide(gms_multisplit_ctx* ctx, const void* kin, const v
 
 gms_status gms_multisplit_set_variant(gms_multisplit_ctx* ctx, int variant) {
     if (!ctx) return GMS_ERR_INVALID_ARG;
-    if (variant != GMS_VARIANT_AUTO && variant != GMS_VARIANT_BMS && variant != GMS_VARIANT_WMS)
+    if (variant != GMS_VARIANT_AUTO && variant != GMS_VARIANT_BMS && variant != GMS_VARIANT_WMS &&
+        variant != GMS_VARIANT_DMS)
         return GMS_ERR_INVALID_ARG;
-    if (variant == GMS_VARIANT_WMS && (ctx->num_buckets < 2 || ctx->num_buckets > kWmsMaxBuckets))
+    if ((variant == GMS_VARIANT_WMS || variant == GMS_VARIANT_DMS) &&
+        (ctx->num_buckets < 2 || ctx->num_buckets > kWmsMaxBuckets))
         return GMS_ERR_UNSUPPORTED_BUCKETS;
     ctx->variant = variant;
     return GMS_SUCCESS;

@@ -38,6 +38,39 @@ struct BatchWork {
     uint32_t tile;
 };
 
+// One instance of elements with a 64-bit key and/or value (KW key words, VW
+// value words per element; the group fixes both).  The identifier is f64 for
+// 64-bit keys and f for 32-bit ones.
+struct BatchWideInst {
+    const uint32_t* kin;
+    const uint32_t* vin;  // NULL without values
+    uint32_t* kout;
+    uint32_t* vout;
+    uint32_t* offs;
+    uint32_t n;
+    uint32_t num_tiles;
+    uint32_t mbase;
+    uint32_t obase;
+    uint32_t nb;
+    AnyF f;
+    AnyF64 f64;
+};
+
+// Launch of one 64-bit group (gms_batch_wide.hip), one workgroup per item.
+struct BatchWideLaunch {
+    const BatchWideInst* inst;
+    const BatchWork* work;  // this group's items
+    uint32_t num_work;
+    uint32_t* H;            // whole concatenated matrix
+    uint32_t m_total;
+    uint32_t keys_total;
+    int log_b;
+    uint32_t kw, vw;
+    bool vec;
+    hipStream_t stream;
+};
+hipError_t launch_batch_wide(const BatchWideLaunch& c, bool post);
+
 // Work order.  Workgroups go to the 8 XCDs round-robin (workgroup i on XCD
 // i mod 8).  Every XCD owns one contiguous range of work items (the split of
 // xcd_tile), so neighbouring tiles of an instance -- which share cache lines

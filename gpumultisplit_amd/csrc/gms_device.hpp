@@ -29,6 +29,18 @@
 
 namespace gms {
 
+// 16-byte load of streamed input: the line is not kept in the caches for
+// reuse (no key or value of a tile is read twice by one kernel).
+__device__ __forceinline__ uint4 ld_stream(const uint4* p) {
+#ifdef GMS_HOST_EMULATION
+    return *p;
+#else
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    return make_uint4(v.x, v.y, v.z, v.w);
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // Bucket identifiers.  Host+device so that the host check entry point runs
 // exactly the integer code the kernels run.  Every identifier returns an id in

@@ -150,6 +150,8 @@ gms_status run_wide_large(Scratch& s, uint32_t* ktmp, uint32_t* vtmp, const uint
                           const gms_bucket_desc64& d, uint32_t* d_offsets, hipStream_t stream);
 // Passes of the radix sort of key_words-word keys (p holds up to 16).
 int wide_sort_passes(uint32_t key_words, uint32_t digit_bits, SortPass* p);
+// Descriptor check for keys of key_bits (32 or 64) bits.
+gms_status check_desc64(uint32_t num_buckets, const gms_bucket_desc64& d, uint32_t key_bits);
 gms_status run_wide_sort(Scratch& s, uint32_t* ktmp, uint32_t* vtmp, const uint32_t* kin,
                          const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint32_t n,
                          uint32_t key_words, uint32_t value_words, uint32_t digit_bits,
@@ -159,9 +161,11 @@ gms_status run_wide_sort(Scratch& s, uint32_t* ktmp, uint32_t* vtmp, const uint3
 // wms_subproblem_keys() keys per wave.
 constexpr uint32_t kWmsMaxBuckets = 64;
 uint32_t wms_subproblem_keys();
+// direct: the DMS postscan (keys scattered from registers, no LDS reorder).
 gms_status run_wms_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin,
                               uint32_t* kout, uint32_t* vout, uint32_t n, uint32_t num_buckets,
-                              gms_bucket_desc d, uint32_t* d_offsets, hipStream_t stream);
+                              gms_bucket_desc d, uint32_t* d_offsets, bool direct,
+                              hipStream_t stream);
 
 gms_status hip_status(hipError_t e);
 bool ranges_overlap(const void* a, size_t an, const void* b, size_t bn);

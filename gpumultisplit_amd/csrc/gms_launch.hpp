@@ -6,22 +6,20 @@
 
 namespace gms {
 
+// Long-tile inputs (LARGE) stream their key loads in the prescan.
 template <int LOG_B, bool KV, bool VEC, bool LARGE, class F>
 hipError_t launch_prescan_v(const MsCall& c, const F& f) {
     constexpr int K = tune_items(LOG_B, KV, LARGE);
-    hipLaunchKernelGGL((ms_prescan_kernel<LOG_B, K, VEC, F>), dim3(c.num_tiles), dim3(kThreads),
-                       0, c.stream, c.kin, c.n, c.num_tiles, c.H, f);
+    hipLaunchKernelGGL((ms_prescan_kernel<LOG_B, K, VEC, LARGE, F>), dim3(c.num_tiles),
+                       dim3(kThreads), 0, c.stream, c.kin, c.n, c.num_tiles, c.H, f);
     return hipGetLastError();
 }
 
-// The long-tile shape is instantiated only for classes where it differs.
 template <int LOG_B, bool KV, class F>
 hipError_t launch_prescan_t(const MsCall& c, const F& f) {
-    if constexpr (tune_items(LOG_B, KV, true) != tune_items(LOG_B, KV, false)) {
-        if (c.large)
-            return c.vec_ok ? launch_prescan_v<LOG_B, KV, true, true>(c, f)
-                            : launch_prescan_v<LOG_B, KV, false, true>(c, f);
-    }
+    if (c.large)
+        return c.vec_ok ? launch_prescan_v<LOG_B, KV, true, true>(c, f)
+                        : launch_prescan_v<LOG_B, KV, false, true>(c, f);
     return c.vec_ok ? launch_prescan_v<LOG_B, KV, true, false>(c, f)
                     : launch_prescan_v<LOG_B, KV, false, false>(c, f);
 }

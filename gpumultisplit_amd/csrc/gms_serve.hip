@@ -94,7 +94,8 @@ class Arena {
 struct Req {
     uint64_t ticket = 0;
     gms_request r{};
-    SortPass pass[8];
+    uint32_t kw = 1, vw = 0;  // key and value words per element
+    SortPass pass[16];
     int np = 1;        // passes
     int next = 0;      // next pass to queue
     int retired = 0;   // passes whose step has completed
@@ -106,30 +107,58 @@ struct Req {
     bool done = false;
     gms_status status = GMS_SUCCESS;
 
-    // The batched multisplit item of pass p.
+    bool wide() const { return kw == 2 || vw == 2; }
+    const uint32_t* kin() const { return static_cast<const uint32_t*>(r.d_key_in); }
+    const uint32_t* vin() const { return static_cast<const uint32_t*>(r.d_value_in); }
+    uint32_t* kout() const { return static_cast<uint32_t*>(r.d_key_out); }
+    uint32_t* vout() const { return static_cast<uint32_t*>(r.d_value_out); }
+
+    // Source and destination words of pass p: the last pass writes the
+    // output, the others alternate with the request's pass buffer, and the
+    // input is never written.
+    void pass_buffers(int p, const uint32_t** sk, const uint32_t** sv, uint32_t** dk,
+                      uint32_t** dv) const {
+        auto dst_k = [&](int q) { return ((np - 1 - q) % 2 == 0) ? kout() : ktmp; };
+        auto dst_v = [&](int q) { return ((np - 1 - q) % 2 == 0) ? vout() : vtmp; };
+        const bool kv = vw != 0;
+        *sk = p == 0 ? kin() : dst_k(p - 1);
+        *sv = kv ? (p == 0 ? vin() : dst_v(p - 1)) : nullptr;
+        *dk = dst_k(p);
+        *dv = kv ? dst_v(p) : nullptr;
+    }
+    // The batched multisplit item of pass p (32-bit elements).
     gms_batch_item item(int p) const {
         if (r.op == GMS_OP_MULTISPLIT)
-            return gms_batch_item{r.d_key_in, r.d_value_in, r.d_key_out, r.d_value_out, r.n,
-                                  r.num_buckets, r.desc, r.d_offsets};
-        auto dst_k = [&](int q) { return ((np - 1 - q) % 2 == 0) ? r.d_key_out : ktmp; };
-        auto dst_v = [&](int q) { return ((np - 1 - q) % 2 == 0) ? r.d_value_out : vtmp; };
-        const bool kv = r.d_value_in != nullptr;
-        const uint32_t* sk = p == 0 ? r.d_key_in : dst_k(p - 1);
-        const uint32_t* sv = kv ? (p == 0 ? r.d_value_in : dst_v(p - 1)) : nullptr;
-        return gms_batch_item{sk, sv, dst_k(p), kv ? dst_v(p) : nullptr, r.n,
-                              1u << pass[p].bits,
+            return gms_batch_item{kin(), vin(), kout(), vout(), r.n, r.num_buckets, r.desc,
+                                  r.d_offsets};
+        const uint32_t *sk, *sv;
+        uint32_t *dk, *dv;
+        pass_buffers(p, &sk, &sv, &dk, &dv);
+        return gms_batch_item{sk, sv, dk, dv, r.n, 1u << pass[p].bits,
                               gms_bucket_desc{GMS_BUCKET_DIGIT, pass[p].shift, pass[p].bits},
                               nullptr};
     }
+    // The same for elements with a 64-bit key or value.
+    BatchWideItem wide_item(int p) const {
+        if (r.op == GMS_OP_MULTISPLIT)
+            return BatchWideItem{kin(), vin(), kout(), vout(), r.n, r.num_buckets, kw, vw, r.desc64,
+                                 r.d_offsets};
+        const uint32_t *sk, *sv;
+        uint32_t *dk, *dv;
+        pass_buffers(p, &sk, &sv, &dk, &dv);
+        return BatchWideItem{sk, sv, dk, dv, r.n, 1u << pass[p].bits, kw, vw,
+                             gms_bucket_desc64{GMS_BUCKET_DIGIT, pass[p].bits, pass[p].shift},
+                             nullptr};
+    }
 };
 
-// Device bytes one pass of n keys with 2^log_b matrix rows can need in a
+// Device bytes one pass of n elements with 2^log_b matrix rows can need in a
 // step (its share of the step's tables; shares add up to at least the step).
-static size_t step_share(uint32_t n, int log_b, uint32_t items_per_thread) {
-    const uint64_t tile = (uint64_t)kThreads * items_per_thread;
+static size_t step_share(uint32_t n, int log_b, uint32_t tile_elems, size_t inst_bytes) {
+    const uint64_t tile = tile_elems;
     const uint64_t tiles = n ? (n + tile - 1) / tile : 1;
     const uint64_t m = ((uint64_t)1 << log_b) * tiles;
-    return round256(sizeof(BatchInst)) + round256(tiles * sizeof(BatchWork)) + round256(m * 4) +
+    return round256(inst_bytes) + round256(tiles * sizeof(BatchWork)) + round256(m * 4) +
            round256(((m + kScanChunk - 1) / kScanChunk + 1) * 4) + 256;
 }
 
@@ -232,7 +261,7 @@ struct gms_server {
         if (r->tmp_bytes) {
             if (!arena.alloc(r->tmp_bytes, &r->tmp_off)) return false;
             r->ktmp = (uint32_t*)(arena.base + r->tmp_off);
-            if (r->r.d_value_in) r->vtmp = r->ktmp + (round256((size_t)r->r.n * 4) / 4);
+            if (r->vw) r->vtmp = r->ktmp + (round256((size_t)r->r.n * 4 * r->kw) / 4);
         }
         reserved += r->reserve;
         active.push_back(r);
@@ -245,24 +274,35 @@ struct gms_server {
     // (they touch only scheduler-owned state), so submitters are not held up.
     bool launch_step(std::unique_lock<std::mutex>& lk) {
         std::vector<gms_batch_item> items;
+        std::vector<BatchWideItem> wides;
         std::vector<std::shared_ptr<Req>> reqs;
         uint64_t keys = 0;
         const uint64_t cap = cfg.max_step_keys ? cfg.max_step_keys : (1ull << 31);
         for (auto& r : active) {
             if (r->next >= r->np) continue;
-            if (!items.empty() && keys + r->r.n > cap) break;
-            items.push_back(r->item(r->next));
+            if (!reqs.empty() && keys + r->r.n > cap) break;
             reqs.push_back(r);
             keys += r->r.n;
         }
-        if (items.empty()) return false;
+        if (reqs.empty()) return false;
+        // the step's items of the first `count` requests
+        auto gather = [&](size_t count) {
+            items.clear();
+            wides.clear();
+            for (size_t i = 0; i < count; ++i) {
+                if (reqs[i]->wide()) wides.push_back(reqs[i]->wide_item(reqs[i]->next));
+                else items.push_back(reqs[i]->item(reqs[i]->next));
+            }
+        };
+        gather(reqs.size());
         lk.unlock();
         gms_status s = GMS_SUCCESS;
         size_t off = 0, bytes = 0;
         bool placed = false;
         hipEvent_t ev = nullptr;
-        while (!items.empty()) {
-            s = eng.plan(items.data(), (uint32_t)items.size(), slot);
+        while (!reqs.empty()) {
+            s = eng.plan(items.data(), (uint32_t)items.size(), wides.data(), (uint32_t)wides.size(),
+                         slot);
             if (s != GMS_SUCCESS) break;
             bytes = eng.device_bytes();
             if (arena.alloc(bytes, &off)) {
@@ -276,12 +316,12 @@ struct gms_server {
             // Try fewer items; a single item that still does not fit fails
             // its request below instead of waiting for space that no
             // retirement will free.
-            if (items.size() == 1) break;
-            items.resize(items.size() / 2);
-            reqs.resize(items.size());
+            if (reqs.size() == 1) break;
+            reqs.resize(reqs.size() / 2);
+            gather(reqs.size());
         }
         keys = 0;
-        for (const auto& it : items) keys += it.n;
+        for (const auto& r : reqs) keys += r->r.n;
         if (s == GMS_SUCCESS && placed) {
             if (!events.empty()) {
                 ev = events.back();
@@ -314,7 +354,7 @@ struct gms_server {
         inflight.push_back(Step{std::move(reqs), off, bytes, ev});
         slot ^= 1;
         ++st.steps;
-        st.step_items += items.size();
+        st.step_items += reqs.size();
         st.keys_processed += keys;
         peak = arena.peak;
         return true;
@@ -413,27 +453,43 @@ gms_status gms_server_submit(gms_server* s, const gms_request* q, uint64_t* tick
     const bool kv = q->d_value_in != nullptr;
     if (kv != (q->d_value_out != nullptr)) return GMS_ERR_INVALID_ARG;
     if (q->n && (!q->d_key_in || !q->d_key_out)) return GMS_ERR_INVALID_ARG;
-    const size_t bytes = (size_t)q->n * 4;
-    if (ranges_overlap(q->d_key_in, bytes, q->d_key_out, bytes) ||
-        (kv && (ranges_overlap(q->d_value_in, bytes, q->d_value_out, bytes) ||
-                ranges_overlap(q->d_key_in, bytes, q->d_value_out, bytes) ||
-                ranges_overlap(q->d_value_in, bytes, q->d_key_out, bytes) ||
-                ranges_overlap(q->d_key_out, bytes, q->d_value_out, bytes))))
+    if ((q->key_bytes != 0 && q->key_bytes != 4 && q->key_bytes != 8) ||
+        (q->value_bytes != 0 && q->value_bytes != 4 && q->value_bytes != 8))
+        return GMS_ERR_INVALID_ARG;
+    r->kw = q->key_bytes == 8 ? 2u : 1u;
+    r->vw = kv ? (q->value_bytes == 8 ? 2u : 1u) : 0u;
+    const size_t kb = (size_t)q->n * 4 * r->kw, vb = (size_t)q->n * 4 * r->vw;
+    auto al = [](const void* p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; };
+    if (!al(q->d_key_in, 4 * r->kw) || !al(q->d_key_out, 4 * r->kw) ||
+        (kv && (!al(q->d_value_in, 4 * r->vw) || !al(q->d_value_out, 4 * r->vw))))
+        return GMS_ERR_INVALID_ARG;
+    if (ranges_overlap(q->d_key_in, kb, q->d_key_out, kb) ||
+        (kv && (ranges_overlap(q->d_value_in, vb, q->d_value_out, vb) ||
+                ranges_overlap(q->d_key_in, kb, q->d_value_out, vb) ||
+                ranges_overlap(q->d_value_in, vb, q->d_key_out, kb) ||
+                ranges_overlap(q->d_key_out, kb, q->d_value_out, vb))))
         return GMS_ERR_ALIAS;
     const uint32_t K = s->cfg.items_per_thread;
+    // tables of one pass: the 32-bit batched tile, or the 64-bit one
+    const uint32_t tile = r->wide() ? wide_tile_elems(r->kw, r->vw) : kThreads * K;
+    const size_t inst = r->wide() ? sizeof(BatchWideInst) : sizeof(BatchInst);
     if (q->op == GMS_OP_MULTISPLIT) {
-        gms_batch_item it{q->d_key_in, q->d_value_in, q->d_key_out, q->d_value_out, q->n,
-                          q->num_buckets, q->desc, q->d_offsets};
-        gms_status st = check_batch_item(it);
+        gms_status st;
+        if (r->wide()) {
+            if (q->num_buckets < 1 || q->num_buckets > 256) return GMS_ERR_UNSUPPORTED_BUCKETS;
+            st = check_desc64(q->num_buckets, q->desc64, 32 * r->kw);
+        } else {
+            st = check_batch_item(r->item(0));
+        }
         if (st != GMS_SUCCESS) return st;
         r->np = 1;
         const int lb = ceil_log2_u32(q->num_buckets);
-        r->reserve = 2 * step_share(q->n, lb < 1 ? 1 : lb, K);
+        r->reserve = 2 * step_share(q->n, lb < 1 ? 1 : lb, tile, inst);
     } else if (q->op == GMS_OP_SORT) {
-        r->np = sort_schedule(q->digit_bits, r->pass);
+        r->np = wide_sort_passes(r->kw, q->digit_bits, r->pass);
         if (r->np == 0) return GMS_ERR_INVALID_ARG;
-        r->tmp_bytes = round256(bytes) * (kv ? 2 : 1);
-        r->reserve = round256(r->tmp_bytes) + 2 * step_share(q->n, 8, K);
+        r->tmp_bytes = round256(kb) + (kv ? round256(vb) : 0);
+        r->reserve = round256(r->tmp_bytes) + 2 * step_share(q->n, 8, tile, inst);
     } else {
         return GMS_ERR_INVALID_ARG;
     }

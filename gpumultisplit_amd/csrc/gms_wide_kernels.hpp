@@ -62,20 +62,19 @@ __device__ __forceinline__ void load_elems(const uint32_t* __restrict__ p, size_
     }
 }
 
-// Prescan: the tile's key words are read as one flat array in 16-byte stripes
-// (the 32-bit prescan's lane-contiguous loads); word 4*(j*256+t)+i belongs to
-// element (4*(j*256+t)+i) / KW, and a 64-bit key never straddles a stripe.
+// Prescan of tile t: the tile's key words are read as one flat array in
+// 16-byte stripes (the 32-bit prescan's lane-contiguous loads); word
+// 4*(j*256+t)+i belongs to element (4*(j*256+t)+i) / KW, and a 64-bit key
+// never straddles a stripe.  Writes column t of the num_tiles-column matrix H.
 template <int LOG_B, int K, int KW, bool VEC, class F>
-__global__ __launch_bounds__(kThreads) void ms_wide_prescan_kernel(const uint32_t* __restrict__ kin,
-                                                                   uint32_t n, uint32_t num_tiles,
-                                                                   uint32_t* __restrict__ H, F f) {
+__device__ __forceinline__ void wide_prescan_tile(const uint32_t* __restrict__ kin, uint32_t n,
+                                                  uint32_t t, uint32_t num_tiles,
+                                                  uint32_t* __restrict__ H, F f, uint32_t* s_h) {
     constexpr int B = 1 << LOG_B;
     constexpr int WPT = K * KW;  // key words per thread
     constexpr uint32_t TILE = kThreads * K;
     static_assert(WPT % 4 == 0, "16-byte stripes");
-    __shared__ __attribute__((aligned(16))) uint32_t s_h[kWaves * B];
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
-    const uint32_t t = xcd_tile(blockIdx.x, num_tiles);
     const uint32_t nvalid = tile_valid(n, t, TILE);
     const size_t base = (size_t)t * TILE * KW;
     uint32_t x[WPT];
@@ -104,6 +103,15 @@ __global__ __launch_bounds__(kThreads) void ms_wide_prescan_kernel(const uint32_
         for (int q = 0; q < kWaves; ++q) s += s_h[q * B + b];
         H[(size_t)b * num_tiles + t] = s;
     }
+}
+
+template <int LOG_B, int K, int KW, bool VEC, class F>
+__global__ __launch_bounds__(kThreads) void ms_wide_prescan_kernel(const uint32_t* __restrict__ kin,
+                                                                   uint32_t n, uint32_t num_tiles,
+                                                                   uint32_t* __restrict__ H, F f) {
+    __shared__ __attribute__((aligned(16))) uint32_t s_h[kWaves * (1 << LOG_B)];
+    wide_prescan_tile<LOG_B, K, KW, VEC>(kin, n, xcd_tile(blockIdx.x, num_tiles), num_tiles, H, f,
+                                         s_h);
 }
 
 template <int LOG_B, int K, int LEVEL_BITS, int KW, int VW, bool FULL, class F>
@@ -160,27 +168,56 @@ __device__ __forceinline__ void wide_postscan_tile(
     }
 }
 
+// LDS of the 64-bit postscan: packed words, then the element words (which
+// reuse the ranking counters' area), bucket deltas and scan scratch.
+template <int LOG_B, int K, int LEVEL_BITS, int KW, int VW>
+struct WideSmem {
+    static constexpr uint32_t TILE = kThreads * K;
+    static constexpr uint32_t CNT = cnt_words(LEVEL_BITS);
+    static constexpr uint32_t PAY = TILE * (KW + VW);
+    static_assert(TILE <= 0x10000u && LOG_B <= 8, "packed word: 8-bit bucket, 16-bit position");
+    uint32_t keys[TILE];
+    uint32_t u[PAY > CNT ? PAY : CNT];
+    uint32_t delta[1 << LOG_B];
+    uint32_t bs[kWaves];
+    uint32_t ws[kWaves];
+};
+
+// Postscan of tile t of an input of n elements, whose bucket b starts at
+// output element g0 (thread b's value) and holds cntb of the tile's elements.
+template <int LOG_B, int K, int LEVEL_BITS, int KW, int VW, bool VEC, class F>
+__device__ __forceinline__ void wide_postscan_item(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, uint32_t t,
+    uint32_t g0, uint32_t cntb, F f, WideSmem<LOG_B, K, LEVEL_BITS, KW, VW>& sm) {
+    constexpr uint32_t TILE = kThreads * K;
+    const uint32_t nvalid = tile_valid(n, t, TILE);
+    const size_t base = (size_t)t * TILE;
+    uint32_t kx[K * KW], vx[K * (VW ? VW : 1)];
+    if (nvalid == TILE) {
+        load_elems<KW, K, VEC, true>(kin, base, TILE, kx);
+        if constexpr (VW) load_elems<VW, K, VEC, true>(vin, base, TILE, vx);
+        wide_postscan_tile<LOG_B, K, LEVEL_BITS, KW, VW, true>(kin, vin, kout, vout, n, base, TILE,
+                                                              g0, cntb, f, sm.keys, sm.u, sm.delta,
+                                                              sm.bs, sm.ws, kx, vx);
+    } else {
+        load_elems<KW, K, false, false>(kin, base, nvalid, kx);
+        if constexpr (VW) load_elems<VW, K, false, false>(vin, base, nvalid, vx);
+        wide_postscan_tile<LOG_B, K, LEVEL_BITS, KW, VW, false>(kin, vin, kout, vout, n, base,
+                                                               nvalid, g0, cntb, f, sm.keys, sm.u,
+                                                               sm.delta, sm.bs, sm.ws, kx, vx);
+    }
+}
+
 template <int LOG_B, int K, int LEVEL_BITS, int KW, int VW, bool VEC, class F>
 __global__ __launch_bounds__(kThreads) void ms_wide_postscan_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, uint32_t num_tiles,
     const uint32_t* __restrict__ G, F f) {
     constexpr int B = 1 << LOG_B;
-    constexpr int NW = KW + VW;
-    constexpr uint32_t TILE = kThreads * K;
-    constexpr uint32_t CNT = cnt_words(LEVEL_BITS);
-    constexpr uint32_t PAY = TILE * NW;
-    static_assert(TILE <= 0x10000u && LOG_B <= 8, "packed word: 8-bit bucket, 16-bit position");
-    __shared__ __attribute__((aligned(16))) uint32_t s_keys[TILE];
-    __shared__ __attribute__((aligned(16))) uint32_t s_u[PAY > CNT ? PAY : CNT];
-    __shared__ uint32_t s_delta[B];
-    __shared__ uint32_t s_bs[kWaves];
-    __shared__ uint32_t s_ws[kWaves];
-
+    __shared__ __attribute__((aligned(16))) WideSmem<LOG_B, K, LEVEL_BITS, KW, VW> sm;
     const uint32_t tid = threadIdx.x;
     const uint32_t t = xcd_tile(blockIdx.x, num_tiles);
-    const uint32_t nvalid = tile_valid(n, t, TILE);
-    const size_t base = (size_t)t * TILE;
     // G[b][t] and the next entry in scan order (n past the end), as in the
     // 32-bit postscan: their difference is the tile's count in bucket b.
     const uint32_t bt = tid < (uint32_t)B ? tid : (uint32_t)B - 1u;
@@ -189,21 +226,8 @@ __global__ __launch_bounds__(kThreads) void ms_wide_postscan_kernel(
     const uint32_t g0 = G[idx];
     const uint32_t g1raw = G[idx + 1 < mtot ? idx + 1 : idx];
     const uint32_t cntb = (idx + 1 < mtot ? g1raw : n) - g0;
-
-    uint32_t kx[K * KW], vx[K * (VW ? VW : 1)];
-    if (nvalid == TILE) {
-        load_elems<KW, K, VEC, true>(kin, base, TILE, kx);
-        if constexpr (VW) load_elems<VW, K, VEC, true>(vin, base, TILE, vx);
-        wide_postscan_tile<LOG_B, K, LEVEL_BITS, KW, VW, true>(kin, vin, kout, vout, n, base, TILE,
-                                                              g0, cntb, f, s_keys, s_u, s_delta,
-                                                              s_bs, s_ws, kx, vx);
-    } else {
-        load_elems<KW, K, false, false>(kin, base, nvalid, kx);
-        if constexpr (VW) load_elems<VW, K, false, false>(vin, base, nvalid, vx);
-        wide_postscan_tile<LOG_B, K, LEVEL_BITS, KW, VW, false>(kin, vin, kout, vout, n, base,
-                                                               nvalid, g0, cntb, f, s_keys, s_u,
-                                                               s_delta, s_bs, s_ws, kx, vx);
-    }
+    wide_postscan_item<LOG_B, K, LEVEL_BITS, KW, VW, VEC>(kin, vin, kout, vout, n, t, g0, cntb, f,
+                                                          sm);
 }
 
 // ---------------------------------------------------------------- launches

@@ -15,6 +15,12 @@
 // Same three stages as the block-wide path on the same bucket-major matrix,
 // with one column per wave subproblem; a block of four waves takes four
 // consecutive subproblems in the XCD-contiguous tile order.
+//
+// DMS, the reference's direct variant (DMS_postscan_4rolls_protected,
+// wms_postscan.cuh:328-505, and its pairs twin wms_postscan_pairs.cuh:357),
+// is the same postscan without the LDS reorder: every key goes from its
+// register straight to global start + running count + rank, so lanes of one
+// store instruction write up to B separate runs.
 #include "gms_internal.hpp"
 #include "gms_kernels.hpp"
 
@@ -57,15 +63,15 @@ __global__ __launch_bounds__(kThreads) void wms_prescan_kernel(const uint32_t* _
     if (lane < B) H[(size_t)lane * num_sub + sub] = cnt[lane];
 }
 
-template <int LOG_B, bool KV, class F>
+template <int LOG_B, bool KV, bool DIRECT, class F>
 __global__ __launch_bounds__(kThreads) void wms_postscan_kernel(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, uint32_t num_sub,
     const uint32_t* __restrict__ G, F f) {
     constexpr uint32_t B = 1u << LOG_B;
     constexpr uint32_t NW = kThreads / 64;
-    __shared__ __attribute__((aligned(16))) uint32_t s_k[NW * kWmsSub];
-    __shared__ __attribute__((aligned(16))) uint32_t s_v[KV ? NW * kWmsSub : 1];
+    __shared__ __attribute__((aligned(16))) uint32_t s_k[DIRECT ? 1 : NW * kWmsSub];
+    __shared__ __attribute__((aligned(16))) uint32_t s_v[(KV && !DIRECT) ? NW * kWmsSub : 1];
     __shared__ uint32_t s_run[kThreads];  // per wave: next local slot of every bucket
     __shared__ uint32_t s_dl[kThreads];   // per wave: global start - local start
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
@@ -106,15 +112,23 @@ __global__ __launch_bounds__(kThreads) void wms_postscan_kernel(
         const uint64_t peers = wave_peers<LOG_B>(d, valid);
         const uint32_t r = lanes_below(peers);
         if (valid) {
-            uint32_t pos = run[d] + r;
-            pos = pos < kWmsSub ? pos : kWmsSub - 1u;  // in bounds even if the counts were not
-            sk[pos] = key[k];
-            if constexpr (KV) sv[pos] = val[k];
+            if constexpr (DIRECT) {
+                uint32_t gp = dl[d] + run[d] + r;
+                gp = gp < n ? gp : n - 1u;  // in bounds even if the counts were not
+                kout[gp] = key[k];
+                if constexpr (KV) vout[gp] = val[k];
+            } else {
+                uint32_t pos = run[d] + r;
+                pos = pos < kWmsSub ? pos : kWmsSub - 1u;  // in bounds even if the counts were not
+                sk[pos] = key[k];
+                if constexpr (KV) sv[pos] = val[k];
+            }
         }
         GMS_WAVE_SYNC();  // every lane has read run[] before the leaders advance it
         if (valid && r == 0u) run[d] += (uint32_t)__builtin_popcountll(peers);
         GMS_WAVE_SYNC();
     }
+    if constexpr (DIRECT) return;
     // Slot-order write-out: consecutive lanes, consecutive slots of a bucket run.
 #pragma unroll
     for (uint32_t j = 0; j < kWmsItems; ++j) {
@@ -130,40 +144,51 @@ __global__ __launch_bounds__(kThreads) void wms_postscan_kernel(
 }
 
 // ---------------------------------------------------------------- launches
+template <int LOG_B, bool KV, bool DIRECT>
+static void wms_launch_post(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                            uint32_t* vout, uint32_t n, uint32_t num_sub, uint32_t* H,
+                            const AnyF& f, uint32_t blocks, hipStream_t stream) {
+    hipLaunchKernelGGL((wms_postscan_kernel<LOG_B, KV, DIRECT, AnyF>), dim3(blocks),
+                       dim3(kThreads), 0, stream, kin, vin, kout, vout, n, num_sub, H, f);
+}
+
 template <int LOG_B>
 static hipError_t wms_launch_b(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
                                uint32_t* vout, uint32_t n, uint32_t num_sub, uint32_t* H,
-                               const AnyF& f, bool prescan, hipStream_t stream) {
+                               const AnyF& f, bool prescan, bool direct, hipStream_t stream) {
     const uint32_t blocks = (num_sub + kThreads / 64 - 1) / (kThreads / 64);
     if (prescan)
         hipLaunchKernelGGL((wms_prescan_kernel<LOG_B, AnyF>), dim3(blocks), dim3(kThreads), 0,
                            stream, kin, n, num_sub, H, f);
+    else if (vin && direct)
+        wms_launch_post<LOG_B, true, true>(kin, vin, kout, vout, n, num_sub, H, f, blocks, stream);
     else if (vin)
-        hipLaunchKernelGGL((wms_postscan_kernel<LOG_B, true, AnyF>), dim3(blocks), dim3(kThreads),
-                           0, stream, kin, vin, kout, vout, n, num_sub, H, f);
+        wms_launch_post<LOG_B, true, false>(kin, vin, kout, vout, n, num_sub, H, f, blocks, stream);
+    else if (direct)
+        wms_launch_post<LOG_B, false, true>(kin, vin, kout, vout, n, num_sub, H, f, blocks, stream);
     else
-        hipLaunchKernelGGL((wms_postscan_kernel<LOG_B, false, AnyF>), dim3(blocks),
-                           dim3(kThreads), 0, stream, kin, vin, kout, vout, n, num_sub, H, f);
+        wms_launch_post<LOG_B, false, false>(kin, vin, kout, vout, n, num_sub, H, f, blocks, stream);
     return hipGetLastError();
 }
 
 static hipError_t wms_launch(int log_b, const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
                              uint32_t* vout, uint32_t n, uint32_t num_sub, uint32_t* H,
-                             const AnyF& f, bool prescan, hipStream_t stream) {
+                             const AnyF& f, bool prescan, bool direct, hipStream_t stream) {
     switch (log_b) {
-        case 1: return wms_launch_b<1>(kin, vin, kout, vout, n, num_sub, H, f, prescan, stream);
-        case 2: return wms_launch_b<2>(kin, vin, kout, vout, n, num_sub, H, f, prescan, stream);
-        case 3: return wms_launch_b<3>(kin, vin, kout, vout, n, num_sub, H, f, prescan, stream);
-        case 4: return wms_launch_b<4>(kin, vin, kout, vout, n, num_sub, H, f, prescan, stream);
-        case 5: return wms_launch_b<5>(kin, vin, kout, vout, n, num_sub, H, f, prescan, stream);
-        case 6: return wms_launch_b<6>(kin, vin, kout, vout, n, num_sub, H, f, prescan, stream);
+        case 1: return wms_launch_b<1>(kin, vin, kout, vout, n, num_sub, H, f, prescan, direct, stream);
+        case 2: return wms_launch_b<2>(kin, vin, kout, vout, n, num_sub, H, f, prescan, direct, stream);
+        case 3: return wms_launch_b<3>(kin, vin, kout, vout, n, num_sub, H, f, prescan, direct, stream);
+        case 4: return wms_launch_b<4>(kin, vin, kout, vout, n, num_sub, H, f, prescan, direct, stream);
+        case 5: return wms_launch_b<5>(kin, vin, kout, vout, n, num_sub, H, f, prescan, direct, stream);
+        case 6: return wms_launch_b<6>(kin, vin, kout, vout, n, num_sub, H, f, prescan, direct, stream);
         default: return hipErrorInvalidValue;
     }
 }
 
 gms_status run_wms_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin,
                               uint32_t* kout, uint32_t* vout, uint32_t n, uint32_t B,
-                              gms_bucket_desc d, uint32_t* d_offsets, hipStream_t stream) {
+                              gms_bucket_desc d, uint32_t* d_offsets, bool direct,
+                              hipStream_t stream) {
     if (B < 2 || B > kWmsMaxBuckets) return GMS_ERR_UNSUPPORTED_BUCKETS;
     if (n == 0) return d_offsets ? hip_status(launch_fill(d_offsets, 0u, B, stream)) : GMS_SUCCESS;
     const int lb = ceil_log2_u32(B);
@@ -171,9 +196,10 @@ gms_status run_wms_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* v
     const size_t m = ((size_t)1 << lb) * ns;
     if (m > s.hist_cap || scan_partials_needed(m) > s.part_cap) return GMS_ERR_CAPACITY;
     const AnyF f = AnyF::make(d.kind, d.param, d.width, B);
-    hipError_t e = wms_launch(lb, kin, vin, kout, vout, n, ns, s.hist, f, true, stream);
+    hipError_t e = wms_launch(lb, kin, vin, kout, vout, n, ns, s.hist, f, true, direct, stream);
     if (e == hipSuccess) e = launch_exclusive_scan(s.hist, s.hist, (uint32_t)m, s.part, stream);
-    if (e == hipSuccess) e = wms_launch(lb, kin, vin, kout, vout, n, ns, s.hist, f, false, stream);
+    if (e == hipSuccess)
+        e = wms_launch(lb, kin, vin, kout, vout, n, ns, s.hist, f, false, direct, stream);
     if (e == hipSuccess && d_offsets) e = launch_gather_offsets(s.hist, ns, B, d_offsets, stream);
     return hip_status(e);
 }

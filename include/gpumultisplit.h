@@ -154,7 +154,7 @@ gms_status gms_multisplit(gms_multisplit_ctx *ctx, const uint32_t *d_key_in,
  * call needs 2^ceil(log2 B) * ceil(n / gms_tile_keys(B, key_value, n)) words
  * of matrix plus about 1/256 of that for the scan, and returns
  * GMS_ERR_CAPACITY beyond the array (2^27 key-value elements in 256 buckets
- * take 22 MiB).  Same results as gms_multisplit.  Every call on one GPU
+ * take 19 MiB).  Same results as gms_multisplit.  Every call on one GPU
  * shares that GPU's array: calls that overlap in time on one GPU (two
  * streams, two host threads) race and give wrong results; calls on one
  * stream are ordered and safe. */
@@ -184,13 +184,19 @@ gms_status gms_multisplit_wide(gms_multisplit_ctx *ctx, const void *d_key_in,
  * block-wide kernels, one tile per workgroup (the reference's BMS path,
  * src/kernels/bms/).  WMS: the wave-level kernels, one subproblem per wave
  * ranked with wave ballots (the reference's warp-wide path,
- * src/kernels/wms/wms_prescan.cuh:21-149, wms_postscan.cuh:20-505,
- * wms_postscan_pairs.cuh:29-538, which its driver main_wms.cu selects by
- * hand).  AUTO (the default) picks per call from the bucket count and n. */
+ * src/kernels/wms/wms_prescan.cuh:21-149, wms_postscan.cuh:20-327,
+ * wms_postscan_pairs.cuh:29-356, which its driver main_wms.cu selects by
+ * hand).  DMS: the wave-level kernels whose postscan writes every key from
+ * its register to its output slot, without the LDS reorder (the reference's
+ * DMS_postscan_4rolls_protected, wms_postscan.cuh:328-505, and
+ * wms_postscan_pairs.cuh:357-538).  AUTO (the default) is BMS: on MI355X it
+ * measured as fast or faster than WMS at every bucket count and size (WMS at
+ * most 5 % ahead, at 2 buckets and up to 2^20 keys). */
 typedef enum gms_variant {
     GMS_VARIANT_AUTO = 0,
     GMS_VARIANT_BMS = 1,
-    GMS_VARIANT_WMS = 2
+    GMS_VARIANT_WMS = 2,
+    GMS_VARIANT_DMS = 3
 } gms_variant;
 gms_status gms_multisplit_set_variant(gms_multisplit_ctx *ctx, int variant);
 
@@ -442,23 +448,50 @@ gms_status gms_batch_plan(gms_batch *b, const gms_batch_item *items, uint32_t co
 
 /* One independent radix sort of a batch: multisplit_sort_{4..8}bit[_pairs]
  * (src/api/multisplit_sort.cuh:320-559, multisplit_sort_pairs.cuh:21-260)
- * with the same digit schedule, as gms_sort; values NULL = keys only.  The
- * outputs double as a pass buffer and the input is never written. */
+ * with the same digit schedule, as gms_sort (as gms_sort_wide for 64-bit
+ * keys); values NULL = keys only.  The outputs double as a pass buffer and
+ * the input is never written. */
 typedef struct gms_batch_sort_item {
-    const uint32_t *d_key_in;
-    const uint32_t *d_value_in;
-    uint32_t *d_key_out;
-    uint32_t *d_value_out;
+    const void *d_key_in;
+    const void *d_value_in;
+    void *d_key_out;
+    void *d_value_out;
     uint32_t n;
     uint32_t digit_bits;    /* 4, 5, 6, 7 or 8 */
+    uint32_t key_bytes;     /* 4 or 8 (0 = 4) */
+    uint32_t value_bytes;   /* 4 or 8 (0 = 4); ignored without values */
 } gms_batch_sort_item;
+
+/* One independent multisplit of elements with 32- or 64-bit keys and 32- or
+ * 64-bit values, as gms_multisplit_wide (the reference's KeyT/ValueT
+ * templates, src/api/multisplit.cuh:81-313): values NULL = key-only;
+ * 1..256 buckets; 64-bit arrays 8-byte aligned.  Shapes with 32-bit keys and
+ * values run as gms_batch_item (the descriptor's parameter then fits 32
+ * bits). */
+typedef struct gms_batch_wide_item {
+    const void *d_key_in;
+    const void *d_value_in;
+    void *d_key_out;
+    void *d_value_out;
+    uint32_t n;
+    uint32_t num_buckets;
+    uint32_t key_bytes;     /* 4 or 8 */
+    uint32_t value_bytes;   /* 4 or 8; ignored without values */
+    gms_bucket_desc64 desc;
+    uint32_t *d_offsets;
+} gms_batch_wide_item;
 
 /* Plans multisplits and radix sorts as one batch: step 0 runs every multisplit
  * and the first digit pass of every sort as one batched multisplit, step p
  * the p-th pass of every sort that has one, in stream order.  The batch owns
- * the sorts' other pass buffer (4 bytes per key, 8 with values).
+ * the sorts' other pass buffer (the size of the sort's keys and values).
  * gms_batch_run, _capture, _launch_graph, _autotune and _stats cover every
- * step.  gms_batch_plan is this call with no sorts. */
+ * step.  gms_batch_plan and gms_batch_plan_mixed are this call without the
+ * 64-bit items (and sorts). */
+gms_status gms_batch_plan_wide(gms_batch *b, const gms_batch_item *items, uint32_t count,
+                               const gms_batch_wide_item *wide, uint32_t num_wide,
+                               const gms_batch_sort_item *sorts, uint32_t num_sorts,
+                               void *stream);
 gms_status gms_batch_plan_mixed(gms_batch *b, const gms_batch_item *items, uint32_t count,
                                 const gms_batch_sort_item *sorts, uint32_t num_sorts,
                                 void *stream);
@@ -489,9 +522,11 @@ gms_status gms_batch_stats(const gms_batch *b, uint64_t *keys, uint32_t *work_it
 /* Request server with continuous batching (no counterpart in the reference) */
 /* ------------------------------------------------------------------------ */
 
-/* A request: one multisplit (as gms_multisplit) or one radix sort (as
- * gms_sort).  Buffers are device memory the caller keeps alive and does not
- * touch until the request has completed. */
+/* A request: one multisplit (as gms_multisplit, or gms_multisplit_wide when
+ * a key or value is 64-bit) or one radix sort (as gms_sort, or
+ * gms_sort_wide).  Buffers are device memory the caller keeps alive and does
+ * not touch until the request has completed; 64-bit arrays are 8-byte
+ * aligned. */
 typedef enum gms_request_op {
     GMS_OP_MULTISPLIT = 0,
     GMS_OP_SORT = 1
@@ -499,15 +534,18 @@ typedef enum gms_request_op {
 
 typedef struct gms_request {
     uint32_t op;                  /* gms_request_op */
-    const uint32_t *d_key_in;
-    const uint32_t *d_value_in;   /* NULL = key-only */
-    uint32_t *d_key_out;
-    uint32_t *d_value_out;
+    const void *d_key_in;
+    const void *d_value_in;       /* NULL = key-only */
+    void *d_key_out;
+    void *d_value_out;
     uint32_t n;
     uint32_t num_buckets;         /* multisplit: 1..256 */
-    gms_bucket_desc desc;         /* multisplit: bucket identifier */
+    gms_bucket_desc desc;         /* multisplit of 32-bit keys and values: identifier */
     uint32_t digit_bits;          /* sort: 4..8 */
     uint32_t *d_offsets;          /* multisplit: B bucket starts (device) or NULL */
+    uint32_t key_bytes;           /* 4 or 8 (0 = 4) */
+    uint32_t value_bytes;         /* 4 or 8 (0 = 4); ignored without values */
+    gms_bucket_desc64 desc64;     /* multisplit with a 64-bit key or value: identifier */
 } gms_request;
 
 typedef struct gms_server_config {

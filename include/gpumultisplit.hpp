@@ -332,8 +332,9 @@ inline uint32_t subproblem_size_wms_key_value(uint32_t num_buckets, uint32_t& si
 }
 
 namespace gms {
-// Block-wide (GMS_VARIANT_BMS), wave-level (GMS_VARIANT_WMS, 2..64 buckets) or
-// automatic (GMS_VARIANT_AUTO) kernels for a context's multisplits.
+// Block-wide (GMS_VARIANT_BMS), wave-level (GMS_VARIANT_WMS, 2..64 buckets),
+// wave-level with direct scatter (GMS_VARIANT_DMS, 2..64 buckets) or automatic
+// (GMS_VARIANT_AUTO, which is BMS) kernels for a context's multisplits.
 inline void set_variant(multisplit_context& context, gms_variant v) {
     check(gms_multisplit_set_variant(context.handle, (int)v), "set_variant");
 }

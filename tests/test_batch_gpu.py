@@ -303,3 +303,156 @@ def test_batch_4096_loguniform_equals_plain_calls(dev):
         np.testing.assert_array_equal(from_dev(kout[s:s + n]), wk)
         np.testing.assert_array_equal(from_dev(vout[s:s + n]), wv)
     gms.multisplit_release_memory(ctx)
+
+
+# ---------------------------------------------------------------------------
+# 64-bit elements in the batch: wide multisplits of every shape and 64-bit
+# sorts beside 32-bit items; every instance against the oracle's 64-bit
+# multisplit / sort.
+# ---------------------------------------------------------------------------
+def _u64_dev(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int64)).to(dev)
+
+
+def _from_dev64(t):
+    return t.cpu().numpy().view(np.uint64)
+
+
+# (n, B, key bytes, value bytes (0 = key-only), kind, param, width)
+WIDE = [(0, 8, 8, 8, oracle.RANGE, 0, 0), (1, 2, 8, 0, oracle.RANGE, 0, 0),
+        (300_001, 256, 8, 8, oracle.RANGE, 0, 0), (70_000, 64, 8, 4, oracle.RANGE, 0, 0),
+        (123_457, 17, 4, 8, oracle.RANGE, 0, 0), (1 << 20, 16, 8, 0, oracle.DIGIT, 40, 4),
+        (50_000, 100, 8, 8, oracle.DELTA, (1 << 64) // 100 + 7, 0),
+        (4096 * 3, 128, 8, 4, oracle.IDENTITY, 0, 0)]
+# (n, digit bits, key bytes, value bytes (0 = keys only))
+WIDE_SORTS = [(200_000, 8, 8, 8), (65_537, 6, 8, 0), (30_000, 5, 8, 4), (100_000, 8, 4, 8)]
+
+
+def _rng64(n, seed):
+    return np.random.default_rng(seed).integers(0, 2**64, size=n, dtype=np.uint64)
+
+
+def _make_wide(dev, seed):
+    import torch
+    items, want = [], []
+    for i, (n, B, kb, vb, kind, param, width) in enumerate(WIDE):
+        keys = _rng64(n, seed + i) if kb == 8 else W.uniform_keys(n, seed=seed + i)
+        if kind == oracle.IDENTITY:
+            keys = keys % np.uint64(B + 9)             # some ids past B-1: clamped
+        vals = None if vb == 0 else (_rng64(n, seed + 50 + i) if vb == 8
+                                     else np.arange(n, dtype=np.uint32))
+        kdt = torch.int64 if kb == 8 else torch.int32
+        kin = (_u64_dev(keys, dev) if kb == 8 else to_dev(keys, dev)) if n else \
+            torch.zeros(1, dtype=kdt, device=dev)
+        kout = torch.full((max(n, 1),), 0x5A, dtype=kdt, device=dev)
+        vin = vout = None
+        if vb:
+            vdt = torch.int64 if vb == 8 else torch.int32
+            vin = (_u64_dev(vals, dev) if vb == 8 else to_dev(vals, dev)) if n else \
+                torch.zeros(1, dtype=vdt, device=dev)
+            vout = torch.full((max(n, 1),), 0x5A, dtype=vdt, device=dev)
+        f = gms.ms_bucket_sort_4bit(param) if kind == oracle.DIGIT else \
+            gms.delta_func(param) if kind == oracle.DELTA else \
+            gms.identity_bucket() if kind == oracle.IDENTITY else gms.range_bucket(B)
+        items.append(gms.BatchWideItem(kin, kout, n, B, f, d_value_in=vin, d_value_out=vout,
+                                       d_offsets=filled_u32(B, dev, 0xFFFFFFFF)))
+        want.append(oracle.multisplit_wide(keys, B, kind, param, vals, width))
+    return items, want
+
+
+def _check_wide(items, want, label):
+    for i, (it, (wk, wv, wo)) in enumerate(zip(items, want)):
+        n = it.n
+        ko = _from_dev64(it.d_key_out) if it.key_bytes == 8 else from_dev(it.d_key_out)
+        np.testing.assert_array_equal(ko[:n], wk, err_msg=f"{label} wide {i} keys")
+        np.testing.assert_array_equal(from_dev(it.d_offsets), wo, err_msg=f"{label} wide {i} offsets")
+        if it.d_value_out is not None:
+            vo = _from_dev64(it.d_value_out) if it.value_bytes == 8 else from_dev(it.d_value_out)
+            np.testing.assert_array_equal(vo[:n], wv, err_msg=f"{label} wide {i} values")
+
+
+def _make_wide_sorts(dev, seed):
+    import torch
+    sorts, want = [], []
+    for i, (n, bits, kb, vb) in enumerate(WIDE_SORTS):
+        keys = _rng64(n, seed + i) if kb == 8 else W.uniform_keys(n, seed=seed + i)
+        if i == 1:
+            keys &= np.uint64(0xFFFF0000000000FF)          # equal digits in most passes
+        vals = None if vb == 0 else (_rng64(n, seed + 40 + i) if vb == 8
+                                     else np.arange(n, dtype=np.uint32))
+        kin = _u64_dev(keys, dev) if kb == 8 else to_dev(keys, dev)
+        kout = torch.full_like(kin, 0x5A)
+        vin = vout = None
+        if vb:
+            vin = _u64_dev(vals, dev) if vb == 8 else to_dev(vals, dev)
+            vout = torch.full_like(vin, 0x5A)
+        sorts.append(gms.BatchSortItem(kin, kout, n, bits, d_value_in=vin, d_value_out=vout))
+        if kb == 8:
+            want.append(oracle.sort_wide(keys, bits, vals))
+        else:   # 32-bit keys, 64-bit values: the stable order applied to the values
+            wk, perm = oracle.sort(keys, bits, np.arange(n, dtype=np.uint32))
+            want.append((wk, vals[perm]))
+    return sorts, want
+
+
+def _check_wide_sorts(sorts, want, label):
+    for i, (s, (wk, wv)) in enumerate(zip(sorts, want)):
+        n = s.n
+        ko = _from_dev64(s.d_key_out) if s.key_bytes == 8 else from_dev(s.d_key_out)
+        np.testing.assert_array_equal(ko[:n], wk, err_msg=f"{label} sort {i} keys")
+        if s.d_value_out is not None:
+            vo = _from_dev64(s.d_value_out) if s.value_bytes == 8 else from_dev(s.d_value_out)
+            np.testing.assert_array_equal(vo[:n], wv, err_msg=f"{label} sort {i} values")
+
+
+@pytest.mark.parametrize("tuning", [(24, 0), (16, 1)])
+def test_batch_wide_items_and_sorts_match_oracle(dev, tuning):
+    """64-bit multisplits and sorts beside 32-bit multisplits and sorts in one
+    batch: every instance equals the oracle; again after a rerun, through
+    graph replays with eager work between them, and after autotuning."""
+    import torch
+    banner(f"batch wide tuning={tuning}")
+    items, want = _make(dev, MIXED[:8], seed=61)
+    wide, wwant = _make_wide(dev, seed=71)
+    wsorts, wswant = _make_wide_sorts(dev, seed=81)
+    sorts, swant, inputs = _make_sorts(dev, seed=91)
+    b = gms.Batch(items, sorts=sorts + wsorts, wide=wide)
+    b.tuning = {"items_per_thread": tuning[0], "persistent": tuning[1]}
+    b.run()
+    torch.cuda.synchronize()
+    _check(items, want, "run")
+    _check_wide(wide, wwant, "run")
+    _check_wide_sorts(wsorts, wswant, "run")
+    _check_sorts(sorts, swant, inputs, "run")
+    b.capture()
+    for rep in range(2):
+        _clear(items)
+        _clear_sorts(sorts + wsorts)
+        for it in wide:
+            it.d_key_out.fill_(0x5A)
+            if it.d_value_out is not None:
+                it.d_value_out.fill_(0x5A)
+        torch.cuda.synchronize()
+        b.replay()
+        torch.cuda.synchronize()
+        _check(items, want, f"replay {rep}")
+        _check_wide(wide, wwant, f"replay {rep}")
+        _check_wide_sorts(wsorts, wswant, f"replay {rep}")
+    b.autotune(reps=1)
+    b.run()
+    torch.cuda.synchronize()
+    _check_wide(wide, wwant, "autotuned")
+    _check_wide_sorts(wsorts, wswant, "autotuned")
+
+
+def test_batch_wide_validation(dev):
+    import torch
+    a = torch.zeros(64, dtype=torch.int64, device=dev)
+    o = torch.zeros(64, dtype=torch.int64, device=dev)
+    with pytest.raises(gms.GmsError):          # 257 buckets
+        gms.Batch([], wide=[gms.BatchWideItem(a, o, 64, 257)])
+    with pytest.raises(gms.GmsError):          # output overlaps the input
+        gms.Batch([], wide=[gms.BatchWideItem(a[:32], a[16:48], 32, 8)])
+    with pytest.raises(gms.GmsError):          # 3-bit digits
+        gms.Batch([], sorts=[gms.BatchSortItem(a, o, 64, 3)])

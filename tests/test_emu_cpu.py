@@ -775,3 +775,211 @@ def test_emulated_noalloc_equals_context_call(emu):
                                       gms.range_bucket(1024).desc(), None, None) == 3
     assert emu.gms_multisplit_noalloc(p(a), None, p(b), None, 4, 1025,
                                       gms.range_bucket(1025).desc(), None, None) == 2
+
+
+# ---------------------------------------------------------------------------
+# Wave-level variants on the emulation: WMS (LDS reorder) and DMS (direct
+# scatter from registers) against the oracle, ragged last subproblem included.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("variant", [2, 3])
+@pytest.mark.parametrize("B,kv", [(2, True), (5, False), (32, True), (64, False)])
+def test_emulated_wave_variants(emu, variant, B, kv):
+    for n in (1, 1000, 4096 + 77):
+        keys = W.keys_for("zipf", n, B, seed=n + B) if B >= 8 else W.uniform_keys(n, seed=n)
+        vals = np.arange(n, dtype=np.uint32)
+        kout = np.full(n, 0x5A5A5A5A, np.uint32)
+        vout = np.full(n, 0x5A5A5A5A, np.uint32)
+        offs = np.zeros(B, np.uint32)
+        h = ctypes.c_void_p()
+        assert emu.gms_multisplit_create(B, ctypes.byref(h)) == 0
+        try:
+            assert emu.gms_multisplit_allocate(h, n, int(kv)) == 0
+            assert emu.gms_multisplit_set_variant(h, variant) == 0
+            assert emu.gms_multisplit(h, p(keys), p(vals) if kv else None, p(kout),
+                                      p(vout) if kv else None, n, gms.range_bucket(B).desc(),
+                                      p(offs), None) == 0
+        finally:
+            emu.gms_multisplit_destroy(h)
+        wk, wv, wo = oracle.multisplit(keys, B, oracle.RANGE, 0, vals if kv else None)
+        np.testing.assert_array_equal(kout, wk, err_msg=f"variant {variant} B={B} n={n}")
+        np.testing.assert_array_equal(offs, wo)
+        if kv:
+            np.testing.assert_array_equal(vout, wv)
+
+
+# ---------------------------------------------------------------------------
+# Batched 64-bit elements on the emulation: wide multisplits of every shape
+# and 64-bit sorts beside 32-bit items, each against the oracle; a 32-bit
+# shape given as a wide item runs as a plain one.
+# ---------------------------------------------------------------------------
+def _u64(n, seed):
+    return np.random.default_rng(seed).integers(0, 2**64, size=n, dtype=np.uint64)
+
+
+# (n, B, key bytes, value bytes (0 = key-only), kind, param)
+BATCH_WIDE_CASES = [(0, 8, 8, 8, oracle.RANGE, 0), (1, 2, 8, 0, oracle.RANGE, 0),
+              (3001, 256, 8, 8, oracle.RANGE, 0), (4097, 64, 8, 4, oracle.RANGE, 0),
+              (2500, 17, 4, 8, oracle.RANGE, 0), (5000, 16, 8, 0, oracle.DIGIT, 40),
+              (777, 32, 4, 4, oracle.DELTA, (1 << 32) // 32 + 1),
+              (3333, 100, 8, 8, oracle.DELTA, (1 << 64) // 100 + 7)]
+# (n, digit bits, key bytes, value bytes (0 = keys only))
+BATCH_WIDE_SORTS = [(3000, 8, 8, 8), (2049, 6, 8, 0), (1500, 5, 8, 4), (2222, 8, 4, 8), (0, 8, 8, 8)]
+
+
+def _wide_desc(B, kind, param):
+    width = 4 if kind == oracle.DIGIT else 0
+    return gms.gms_bucket_desc64(kind, width, param), width
+
+
+@pytest.mark.parametrize("with_plain", [False, True])
+def test_emulated_batch_wide_items_and_sorts(emu, with_plain):
+    keep, witems, wwant = [], [], []
+    for i, (n, B, kb, vb, kind, param) in enumerate(BATCH_WIDE_CASES):
+        keys = _u64(n, 10 + i) if kb == 8 else W.uniform_keys(n, seed=10 + i)
+        vals = (None if vb == 0 else (_u64(n, 50 + i) if vb == 8
+                                      else np.arange(n, dtype=np.uint32)))
+        kout = np.full(max(n, 1), 0x5A, np.uint64 if kb == 8 else np.uint32)
+        vout = None if vb == 0 else np.full(max(n, 1), 0x5A, np.uint64 if vb == 8 else np.uint32)
+        offs = np.full(B, 0xFFFFFFFF, np.uint32)
+        desc, width = _wide_desc(B, kind, param)
+        keep.append((keys, vals, kout, vout, offs))
+        witems.append(gms.gms_batch_wide_item(p(keys) if n else None, p(vals), p(kout), p(vout), n, B,
+                                              kb, vb, desc, p(offs)))
+        wwant.append(oracle.multisplit_wide(keys, B, kind, param, vals, width))
+    skeep, sitems, swant = [], [], []
+    for i, (n, bits, kb, vb) in enumerate(BATCH_WIDE_SORTS):
+        keys = _u64(n, 80 + i) if kb == 8 else W.uniform_keys(n, seed=80 + i)
+        if i == 1:
+            keys &= np.uint64(0xFFFF0000000000FF)   # equal digits in most passes
+        vals = (None if vb == 0 else (_u64(n, 90 + i) if vb == 8
+                                      else np.arange(n, dtype=np.uint32)))
+        kout = np.full(max(n, 1), 0x5A, keys.dtype)
+        vout = None if vb == 0 else np.full(max(n, 1), 0x5A, vals.dtype)
+        skeep.append((keys, keys.copy(), vals, kout, vout))
+        sitems.append(gms.gms_batch_sort_item(p(keys) if n else None, p(vals), p(kout), p(vout), n,
+                                              bits, kb, vb if vb else 4))
+        if kb == 8:
+            swant.append(oracle.sort_wide(keys, bits, vals))
+        else:   # 32-bit keys, 64-bit values: the stable order applied to the values
+            wk, perm = oracle.sort(keys, bits, np.arange(n, dtype=np.uint32))
+            swant.append((wk, vals[perm]))
+    pkeep, pitems, pwant = [], [], []
+    if with_plain:
+        for i, (n, B, f, kind, param, kv) in enumerate(BATCH_CASES[3:7]):
+            keys = W.uniform_keys(n, seed=120 + i)
+            vals = np.arange(n, dtype=np.uint32)
+            kout = np.full(max(n, 1), 0x5A5A5A5A, np.uint32)
+            vout = np.full(max(n, 1), 0x5A5A5A5A, np.uint32)
+            offs = np.full(B, 0xFFFFFFFF, np.uint32)
+            pkeep.append((keys, vals, kout, vout, offs, kv))
+            pitems.append(gms.gms_batch_item(p(keys), p(vals) if kv else None, p(kout),
+                                             p(vout) if kv else None, n, B, f.desc(), p(offs)))
+            pwant.append(oracle.multisplit(keys, B, kind, param, vals if kv else None))
+    h = ctypes.c_void_p()
+    assert emu.gms_batch_create(ctypes.byref(h)) == 0
+    try:
+        parr = (gms.gms_batch_item * max(len(pitems), 1))(*pitems)
+        warr = (gms.gms_batch_wide_item * len(witems))(*witems)
+        sarr = (gms.gms_batch_sort_item * len(sitems))(*sitems)
+        assert emu.gms_batch_plan_wide(h, parr, len(pitems), warr, len(witems), sarr, len(sitems),
+                                       None) == 0
+        assert emu.gms_batch_run(h, None) == 0
+    finally:
+        emu.gms_batch_destroy(h)
+    for i, ((keys, vals, kout, vout, offs), (wk, wv, wo)) in enumerate(zip(keep, wwant)):
+        n = keys.size
+        np.testing.assert_array_equal(kout[:n], wk, err_msg=f"wide item {i} keys")
+        np.testing.assert_array_equal(offs, wo, err_msg=f"wide item {i} offsets")
+        if vals is not None:
+            np.testing.assert_array_equal(vout[:n], wv, err_msg=f"wide item {i} values")
+    for i, ((keys, keys0, vals, kout, vout), (wk, wv)) in enumerate(zip(skeep, swant)):
+        n = keys.size
+        np.testing.assert_array_equal(keys, keys0, err_msg=f"wide sort {i} input written")
+        np.testing.assert_array_equal(kout[:n], wk, err_msg=f"wide sort {i} keys")
+        if vals is not None:
+            np.testing.assert_array_equal(vout[:n], wv, err_msg=f"wide sort {i} values")
+    for i, ((keys, vals, kout, vout, offs, kv), (wk, wv, wo)) in enumerate(zip(pkeep, pwant)):
+        n = keys.size
+        np.testing.assert_array_equal(kout[:n], wk, err_msg=f"plain item {i} keys")
+        np.testing.assert_array_equal(offs, wo, err_msg=f"plain item {i} offsets")
+        if kv:
+            np.testing.assert_array_equal(vout[:n], wv, err_msg=f"plain item {i} values")
+
+
+def test_emulated_batch_wide_validation(emu):
+    a = np.zeros(64, np.uint64)
+    b = np.zeros(64, np.uint64)
+    h = ctypes.c_void_p()
+    assert emu.gms_batch_create(ctypes.byref(h)) == 0
+    rng = gms.gms_bucket_desc64(0, 0, 0)
+
+    def plan(*items):
+        arr = (gms.gms_batch_wide_item * len(items))(*items)
+        return emu.gms_batch_plan_wide(h, None, 0, arr, len(items), None, 0, None)
+    try:
+        W_ = gms.gms_batch_wide_item
+        assert plan(W_(p(a), None, p(b), None, 64, 257, 8, 0, rng, None)) == 2      # buckets
+        assert plan(W_(p(a), None, p(b), None, 64, 8, 2, 0, rng, None)) == 1        # key bytes
+        assert plan(W_(p(a) + 4, None, p(b), None, 32, 8, 8, 0, rng, None)) == 1    # alignment
+        assert plan(W_(p(a), p(a), p(b), None, 64, 8, 8, 8, rng, None)) == 1        # value out
+        assert plan(W_(p(a), None, p(a) + 256, None, 64, 8, 8, 0, rng, None)) == 7  # in/out overlap
+        assert plan(W_(p(a), None, p(b), None, 32, 8, 8, 0, gms.gms_bucket_desc64(2, 4, 64),
+                       None)) == 1                                                   # digit shift
+        assert plan(W_(p(a), None, p(b), None, 64, 8, 8, 0, rng, None)) == 0
+    finally:
+        emu.gms_batch_destroy(h)
+
+
+# ---------------------------------------------------------------------------
+# 64-bit requests on the emulated server: multisplits and sorts of every
+# element shape, interleaved with 32-bit ones, each against the oracle.
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("max_inflight", [1, 2])
+def test_emulated_server_wide_requests(emu, max_inflight):
+    g = W.rng(700 + max_inflight)
+    reqs = []
+    shapes = [(8, 0), (8, 4), (8, 8), (4, 8), (4, 4), (4, 0)]
+    for i in range(16):
+        kb, vb = shapes[i % len(shapes)]
+        n = int(g.integers(0, 6000))
+        keys = _u64(n, 900 + i) if kb == 8 else W.uniform_keys(n, seed=900 + i)
+        vals = None if vb == 0 else (_u64(n, 950 + i) if vb == 8 else np.arange(n, dtype=np.uint32))
+        kout = np.full(max(n, 1), 0x5A, keys.dtype)
+        vout = None if vb == 0 else np.full(max(n, 1), 0x5A, vals.dtype)
+        if i % 2:
+            bits = (8, 5, 6)[i % 3]
+            req = gms.gms_request(1, p(keys) if n else None, p(vals), p(kout), p(vout), n, 0,
+                                  gms.gms_bucket_desc(0, 0, 0), bits, None, kb, vb or 4,
+                                  gms.gms_bucket_desc64(0, 0, 0))
+            if kb == 8:
+                wk, wv = oracle.sort_wide(keys, bits, vals)
+            else:
+                wk, perm = oracle.sort(keys, bits, np.arange(n, dtype=np.uint32))
+                wv = None if vals is None else vals[perm]
+            want = (wk, wv, None)
+            offs = None
+        else:
+            B = (2, 16, 64, 256, 100)[i % 5]
+            offs = np.full(B, 0xFFFFFFFF, np.uint32)
+            desc32 = gms.range_bucket(B).desc() if kb == 4 and vb != 8 else gms.gms_bucket_desc(0, 0, 0)
+            req = gms.gms_request(0, p(keys) if n else None, p(vals), p(kout), p(vout), n, B, desc32,
+                                  0, p(offs), kb, vb or 4, gms.range_bucket(B).desc64())
+            want = oracle.multisplit_wide(keys, B, oracle.RANGE, 0, vals, 0)
+        reqs.append((req, (keys, vals, kout, vout, offs), want, n))
+    h = _emu_server(emu, 64 << 20, max_inflight)
+    try:
+        tickets = []
+        for req, *_ in reqs:
+            t = ctypes.c_uint64()
+            assert emu.gms_server_submit(h, ctypes.byref(req), ctypes.byref(t)) == 0
+            tickets.append(t.value)
+        for t in tickets:
+            assert emu.gms_server_wait(h, t, None) == 0
+    finally:
+        assert emu.gms_server_destroy(h) == 0
+    for i, (_req, (keys, vals, kout, vout, offs), (wk, wv, wo), n) in enumerate(reqs):
+        np.testing.assert_array_equal(kout[:n], wk, err_msg=f"request {i} keys")
+        if vals is not None:
+            np.testing.assert_array_equal(vout[:n], wv, err_msg=f"request {i} values")
+        if offs is not None:
+            np.testing.assert_array_equal(offs, wo, err_msg=f"request {i} offsets")

@@ -154,3 +154,61 @@ def test_server_served_workload_matches_alone(dev):
     print(f"p50 {S.percentile(lats, 50):.1f} us p99 {S.percentile(lats, 99):.1f} us wall {wall:.3f} s")
     for i, r in enumerate(recs):
         _same_as_alone(dev, r, i)
+
+
+def test_server_wide_requests_match_oracle(dev):
+    """64-bit requests of every element shape -- multisplits and sorts --
+    submitted together with 32-bit ones; each result equals the oracle's
+    64-bit multisplit or sort (32-bit keys carrying 64-bit values: the oracle's
+    stable order applied to the values)."""
+    import torch
+    banner("server wide requests")
+    g = np.random.default_rng(77)
+    shapes = [(8, 0), (8, 4), (8, 8), (4, 8), (4, 4), (4, 0)]
+    recs = []
+
+    def dev_arr(a):
+        t = torch.from_numpy(np.ascontiguousarray(a).view(np.int64 if a.dtype == np.uint64 else np.int32))
+        return t.to(dev)
+
+    for i in range(24):
+        kb, vb = shapes[i % len(shapes)]
+        n = int(2 ** g.uniform(0, 20))
+        keys = (g.integers(0, 2**64, size=n, dtype=np.uint64) if kb == 8
+                else g.integers(0, 2**32, size=n, dtype=np.uint64).astype(np.uint32))
+        vals = None if vb == 0 else (g.integers(0, 2**64, size=n, dtype=np.uint64) if vb == 8
+                                     else np.arange(n, dtype=np.uint32))
+        kin = dev_arr(keys)
+        kout = torch.full_like(kin, 0x5A)
+        vin = None if vals is None else dev_arr(vals)
+        vout = None if vin is None else torch.full_like(vin, 0x5A)
+        if i % 2:
+            bits = (8, 6, 5)[i % 3]
+            req = gms.sort_request(kin, kout, n, bits, d_value_in=vin, d_value_out=vout)
+            if kb == 8:
+                wk, wv = oracle.sort_wide(keys, bits, vals)
+            else:
+                wk, perm = oracle.sort(keys, bits, np.arange(n, dtype=np.uint32))
+                wv = None if vals is None else vals[perm]
+            wo, offs = None, None
+        else:
+            B = (2, 16, 64, 256)[i % 4]
+            offs = torch.full((B,), -1, dtype=torch.int32, device=dev)
+            req = gms.multisplit_request(kin, kout, n, B, gms.range_bucket(B), d_value_in=vin,
+                                         d_value_out=vout, d_offsets=offs)
+            wk, wv, wo = oracle.multisplit_wide(keys, B, oracle.RANGE, 0, vals, 0)
+        # the request holds raw pointers: every buffer stays referenced here
+        # until the request has completed
+        recs.append((req, kout, vout, offs, wk, wv, wo, n, kb, vb, kin, vin))
+    with gms.Server(1 << 30, max_inflight_steps=2) as srv:
+        tickets = [srv.submit(r[0]) for r in recs]
+        for t in tickets:
+            srv.wait(t)
+    for i, (_req, kout, vout, offs, wk, wv, wo, n, kb, vb, _kin, _vin) in enumerate(recs):
+        ko = kout.cpu().numpy().view(np.uint64 if kb == 8 else np.uint32)
+        np.testing.assert_array_equal(ko[:n], wk, err_msg=f"request {i} keys")
+        if vout is not None:
+            vo = vout.cpu().numpy().view(np.uint64 if vb == 8 else np.uint32)
+            np.testing.assert_array_equal(vo[:n], wv, err_msg=f"request {i} values")
+        if offs is not None:
+            np.testing.assert_array_equal(from_dev(offs), wo, err_msg=f"request {i} offsets")

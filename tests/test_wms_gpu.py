@@ -1,5 +1,6 @@
-"""GPU parity of the wave-level multisplit variant (WMS: one subproblem per
-wave, ballot ranking; the reference's warp-wide path, src/kernels/wms/),
+"""GPU parity of the wave-level multisplit variants (WMS: one subproblem per
+wave, ballot ranking; DMS: the same with the keys scattered from registers,
+no LDS reorder; the reference's warp-wide path, src/kernels/wms/),
 bit-exact against the oracle and against the block-wide variant on the same
 inputs, for every bucket count up to 64, key-only and key-value."""
 import numpy as np
@@ -33,13 +34,17 @@ def run_variant(dev, keys, B, f, kv, variant):
     return res
 
 
+WAVE_VARIANTS = [gms.VARIANT_WMS, gms.VARIANT_DMS]
+
+
+@pytest.mark.parametrize("variant", WAVE_VARIANTS)
 @pytest.mark.parametrize("B", [2, 3, 4, 8, 16, 17, 32, 33, 64])
 @pytest.mark.parametrize("kv", [False, True])
-def test_wms_matches_oracle(dev, B, kv):
+def test_wms_matches_oracle(dev, B, kv, variant):
     for n in (0, 1, 1023, 1025, 4096 * 3 + 1000, 1_000_003):
-        banner(f"wms B={B} kv={kv} n={n}")
+        banner(f"variant {variant} B={B} kv={kv} n={n}")
         keys = W.uniform_keys(n, seed=B * 31 + n)
-        got = run_variant(dev, keys, B, gms.range_bucket(B), kv, gms.VARIANT_WMS)
+        got = run_variant(dev, keys, B, gms.range_bucket(B), kv, variant)
         wk, wv, wo = oracle.multisplit(keys, B, oracle.RANGE, 0,
                                        np.arange(n, dtype=np.uint32) if kv else None)
         np.testing.assert_array_equal(got[0], wk)
@@ -48,18 +53,20 @@ def test_wms_matches_oracle(dev, B, kv):
             np.testing.assert_array_equal(got[1], wv)
 
 
+@pytest.mark.parametrize("variant", WAVE_VARIANTS)
 @pytest.mark.parametrize("dist", ["zipf", "onehot90"])
 @pytest.mark.parametrize("B", [8, 64])
-def test_wms_equals_bms_on_skewed_keys(dev, dist, B):
+def test_wms_equals_bms_on_skewed_keys(dev, dist, B, variant):
     n = (1 << 22) + 17
     keys = W.keys_for(dist, n, B, seed=B)
-    a = run_variant(dev, keys, B, gms.range_bucket(B), True, gms.VARIANT_WMS)
+    a = run_variant(dev, keys, B, gms.range_bucket(B), True, variant)
     b = run_variant(dev, keys, B, gms.range_bucket(B), True, gms.VARIANT_BMS)
     for x, y in zip(a, b):
         np.testing.assert_array_equal(x, y)
 
 
-def test_wms_other_identifiers(dev):
+@pytest.mark.parametrize("variant", WAVE_VARIANTS)
+def test_wms_other_identifiers(dev, variant):
     n = 777_777
     for B, f, kind, param, keys in [
             (64, gms.delta_func(1 << 26), oracle.DELTA, 1 << 26, W.uniform_keys(n, seed=1)),
@@ -67,7 +74,7 @@ def test_wms_other_identifiers(dev):
             (50, gms.identity_bucket(), oracle.IDENTITY, 0, W.uniform_keys(n, seed=3) % np.uint32(60)),
             (2, gms.test_bucket(1), oracle.LOWBITS, 1, W.uniform_keys(n, seed=4))]:
         keys = keys.astype(np.uint32)
-        got = run_variant(dev, keys, B, f, True, gms.VARIANT_WMS)
+        got = run_variant(dev, keys, B, f, True, variant)
         wk, wv, wo = oracle.multisplit(keys, B, kind, param, np.arange(n, dtype=np.uint32))
         np.testing.assert_array_equal(got[0], wk)
         np.testing.assert_array_equal(got[1], wv)
@@ -81,14 +88,19 @@ def test_wms_sizes_and_rejection(dev):
     ctx = gms.multisplit_context(256)
     with pytest.raises(gms.GmsError):
         gms.set_variant(ctx, gms.VARIANT_WMS)
+    with pytest.raises(gms.GmsError):
+        gms.set_variant(ctx, gms.VARIANT_DMS)
+    with pytest.raises(gms.GmsError):
+        gms.set_variant(gms.multisplit_context(8), 4)
 
 
 @pytest.mark.slow
-def test_wms_2e27_kv_32(dev):
+@pytest.mark.parametrize("variant", WAVE_VARIANTS)
+def test_wms_2e27_kv_32(dev, variant):
     n, B = 1 << 27, 32
-    banner(f"wms B={B} n={n}")
+    banner(f"variant {variant} B={B} n={n}")
     keys = W.uniform_keys(n, seed=27)
-    got = run_variant(dev, keys, B, gms.range_bucket(B), True, gms.VARIANT_WMS)
+    got = run_variant(dev, keys, B, gms.range_bucket(B), True, variant)
     wk, wv, wo = oracle.multisplit(keys, B, oracle.RANGE, 0, np.arange(n, dtype=np.uint32))
     np.testing.assert_array_equal(got[0], wk)
     np.testing.assert_array_equal(got[1], wv)
