@@ -58,8 +58,11 @@ def bench_served(args, torch, gms, dev):
     at once (median of three runs after one untimed full-size run, whose time
     is reported as first_run_ms); then Poisson arrivals at 30/60/90 % of it,
     three runs each, reporting throughput and p50/p99 latency of the run
-    with the median p99.  The same requests as back-to-back plain calls give
-    the comparison throughput."""
+    with the median p99; then the steady state: the request set replayed at
+    each rate of a ladder (30 % .. 130 % of capacity) for at least 0.5 s of
+    arrivals, and the highest rate whose achieved rate is at least 98 % of
+    the offered one.  The same requests as back-to-back plain calls give the
+    comparison throughput."""
     import numpy as np
     from gpumultisplit_amd import served as S
     specs = S.request_specs(args.served_requests, seed=77)
@@ -102,7 +105,7 @@ def bench_served(args, torch, gms, dev):
         # least 0.5 s of arrivals, over three copies of the output buffers.
         sets = S.output_copies(gms, torch, recs, 3)
         ladder = []
-        for j, frac in enumerate((0.3, 0.6, 0.8, 0.9, 0.95, 1.0, 1.1)):
+        for j, frac in enumerate((0.3, 0.6, 0.8, 0.9, 1.0, 1.1, 1.2, 1.3)):
             r = S.run_steady(srv, sets, frac * cap, 0.5, seed=300 + j)
             ladder.append({"offered_req_per_s": round(frac * cap, 1),
                            "achieved_req_per_s": round(r["requests"] / r["wall_s"], 1),
@@ -166,6 +169,11 @@ def bench_served(args, torch, gms, dev):
     return out
 
 
+def _check_status(st, gms, what):
+    if st != 0:
+        raise gms.GmsError(st, what)
+
+
 def bench_batched(args, torch, gms, W, dev):
     """Batched form: 4096 independent key-value multisplits, sizes round(2^U),
     U ~ U[8, 22], 64 buckets by range, uniform keys, all in HBM.  Times the
@@ -227,6 +235,58 @@ def bench_batched(args, torch, gms, W, dev):
     gms.multisplit_release_memory(ctx)
     st = batch.stats()
     del batch, items
+
+    # Batched radix sorts: 1024 key-value 8-bit-digit sorts of uneven sizes as
+    # one batch (4 steps), against the same sorts as back-to-back gms_sort calls.
+    scount = 1024
+    ssizes = W.log_uniform_sizes(scount, 8, 20, seed=1024)
+    sstarts = np.zeros(scount, dtype=np.int64)
+    for i in range(1, scount):
+        sstarts[i] = (sstarts[i - 1] + ssizes[i - 1] + 3) // 4 * 4
+    stotal = int(ssizes.sum())
+    print(f"[bench] batched sorts: {scount} sorts, {stotal} keys", file=sys.stderr, flush=True)
+    sorts = [gms.BatchSortItem(kin[s:s + n], kout[s:s + n], int(n), 8, d_value_in=vin[s:s + n],
+                               d_value_out=vout[s:s + n]) for s, n in zip(sstarts, ssizes)]
+    sbatch = gms.Batch([], sorts=sorts)
+    stuned = sbatch.autotune(reps=1)
+    sbatch.capture()
+    sbatch.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sbatch.replay()
+    torch.cuda.synchronize()
+    ms_sbatch = (time.perf_counter() - t0) * 1e3 / reps
+    sctx = gms.ms_sort_context()
+    _check_status(L.gms_sort_allocate(sctx.handle, int(ssizes.max()), 8, 1), gms, "gms_sort_allocate")
+    scalls = [(kin[s:s + n].data_ptr(), kout[s:s + n].data_ptr(), vin[s:s + n].data_ptr(),
+               vout[s:s + n].data_ptr(), int(n)) for s, n in zip(sstarts, ssizes)]
+
+    def plain_sorts():
+        for kp, kop, vp, vop, n in scalls:
+            _check_status(L.gms_sort(sctx.handle, kp, kop, vp, vop, n, 8, stream), gms, "gms_sort")
+
+    plain_sorts()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain_sorts()
+    torch.cuda.synchronize()
+    ms_splain = (time.perf_counter() - t0) * 1e3 / reps
+    sst = sbatch.stats()
+    del sbatch, sorts
+    batched_sorts = {
+        "workload": f"{scount} key-value radix sorts (8-bit digits, 4 passes), sizes round(2^U) "
+                    "U~U[8,20]",
+        "total_keys": stotal, "sorts": scount,
+        "ms_per_batch": round(ms_sbatch, 4), "gkeys_per_s": round(stotal / ms_sbatch / 1e6, 3),
+        "plain_back_to_back_ms": round(ms_splain, 4),
+        "plain_back_to_back_gkeys_per_s": round(stotal / ms_splain / 1e6, 3),
+        "speedup_vs_plain": round(ms_splain / ms_sbatch, 3),
+        "hbm_fraction": round(stotal * 80 / (ms_sbatch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "launches_per_batch": sst["launches"],
+        "tuning": {"items_per_thread": stuned["items_per_thread"], "persistent": stuned["persistent"]},
+    }
     return {
         "workload": f"{count} key-value multisplits, sizes round(2^U) U~U[8,22], {B} buckets (range), "
                     "16-byte-aligned instance buffers",
@@ -239,6 +299,7 @@ def bench_batched(args, torch, gms, W, dev):
         "launches_per_batch": st["launches"], "work_items": st["work_items"],
         "tuning": {"items_per_thread": tuned["items_per_thread"], "persistent": tuned["persistent"]},
         "hipgraph": True, "reps": reps,
+        "sorts": batched_sorts,
     }
 
 

@@ -104,8 +104,15 @@ gms_status scratch_alloc(Scratch& s, size_t hist_elems) {
         scratch_free(s);
         return GMS_ERR_ALLOC;
     }
+    const size_t sw = scan_chunks(he);
+    if (hipMalloc((void**)&s.status, sw * sizeof(uint64_t)) != hipSuccess) {
+        s.status = nullptr;
+        scratch_free(s);
+        return GMS_ERR_ALLOC;
+    }
     s.hist_cap = he;
     s.part_cap = pe;
+    s.status_cap = sw;
     return GMS_SUCCESS;
 }
 
@@ -113,8 +120,10 @@ void scratch_free(Scratch& s) {
     if (s.hist) (void)hipFree(s.hist);
     if (s.part) (void)hipFree(s.part);
     if (s.offs) (void)hipFree(s.offs);
+    if (s.status) (void)hipFree(s.status);
     s.hist = s.part = s.offs = nullptr;
-    s.hist_cap = s.part_cap = 0;
+    s.status = nullptr;
+    s.hist_cap = s.part_cap = s.status_cap = 0;
 }
 
 hipEvent_t* Profiler::next(hipStream_t) {
@@ -210,6 +219,19 @@ static hipError_t with_functor(uint32_t B, gms_bucket_desc d, Fn&& fn) {
 
 static bool vec_aligned(const void* p) { return p == nullptr || (((uintptr_t)p) & 15u) == 0; }
 
+// The plain path scans the matrix in one launch when the scratch has status
+// words for it (cleared by the prescan), else with the two-launch scan.
+static bool single_scan(const Scratch& s, size_t m) {
+    return s.status && scan_chunks(m) <= s.status_cap;
+}
+static bool scan_fits(const Scratch& s, size_t m) {
+    return single_scan(s, m) || scan_partials_needed(m) <= s.part_cap;
+}
+static hipError_t scan_matrix(const Scratch& s, uint32_t m, hipStream_t stream) {
+    if (single_scan(s, m)) return launch_scan_single(s.hist, s.hist, m, s.status, stream);
+    return launch_exclusive_scan(s.hist, s.hist, m, s.part, stream);
+}
+
 // The launch description of the plain stages for n keys in B buckets, or
 // GMS_ERR_CAPACITY when the scratch is too small for it.
 static gms_status plan_call(Scratch& s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
@@ -219,9 +241,13 @@ static gms_status plan_call(Scratch& s, const uint32_t* kin, const uint32_t* vin
     const uint32_t tile = tile_keys(lb, kv, n);
     const uint32_t nt = (uint32_t)(((size_t)n + tile - 1) / tile);
     const size_t m = ((size_t)1 << lb) * nt;
-    if (m > s.hist_cap || scan_partials_needed(m) > s.part_cap) return GMS_ERR_CAPACITY;
+    if (m > s.hist_cap || !scan_fits(s, m)) return GMS_ERR_CAPACITY;
     c = MsCall{kin, vin, kout, vout, n, lb, kv, n >= kLongTileMin, nt, s.hist,
                vec_aligned(kin) && vec_aligned(vin), stream};
+    if (single_scan(s, m)) {
+        c.scan_status = s.status;
+        c.scan_chunks = scan_chunks(m);
+    }
     return GMS_SUCCESS;
 }
 
@@ -260,7 +286,7 @@ static gms_status run_stages(Scratch& s, const uint32_t* kin, const uint32_t* vi
     if (ev) e = hipEventRecord(ev[0], stream);
     if (e == hipSuccess) e = with_f([&](auto f) { return launch_prescan(c, f); });
     if (e == hipSuccess && ev) e = hipEventRecord(ev[1], stream);
-    if (e == hipSuccess) e = launch_exclusive_scan(s.hist, s.hist, m, s.part, stream);
+    if (e == hipSuccess) e = scan_matrix(s, m, stream);
     if (e == hipSuccess && ev) e = hipEventRecord(ev[2], stream);
     if (e == hipSuccess) e = with_f([&](auto f) { return launch_postscan(c, f); });
     if (e == hipSuccess && ev) e = hipEventRecord(ev[3], stream);
@@ -383,11 +409,14 @@ static gms_status stage_plan(Scratch& s, uint32_t capacity, int allocated, uint3
 }  // namespace gms
 
 // ---------------------------------------------------------------- no-alloc form
-// Scratch of gms_multisplit_noalloc: the per-tile count matrix, then the scan
-// partials.  One copy per GPU, held until the process ends.  In host code
-// `scratch` names a host-side stand-in, never the device memory: host code
-// uses only the address hipGetSymbolAddress returns on each device.
+// Scratch of gms_multisplit_noalloc: the single-pass scan's status words
+// (cleared by each call's prescan), then the per-tile count matrix.  One copy
+// per GPU, held until the process ends.  In host code `scratch` names a
+// host-side stand-in, never the device memory: host code uses only the
+// address hipGetSymbolAddress returns on each device.
 constexpr size_t kNoAllocWords = GMS_NOALLOC_SCRATCH_BYTES / sizeof(uint32_t);
+constexpr size_t kNoAllocStatus = (kNoAllocWords + kScanChunk - 1) / kScanChunk;  // 64-bit words
+constexpr size_t kNoAllocHead = (2 * kNoAllocStatus + 63) / 64 * 64;             // 32-bit words
 static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
 
 namespace {
@@ -603,26 +632,24 @@ gms_status gms_multisplit_noalloc(const uint32_t* kin, const uint32_t* vin, uint
     if (num_buckets < 1 || num_buckets > kSinglePassBuckets) return GMS_ERR_UNSUPPORTED_BUCKETS;
     gms_status st = check_desc(num_buckets, d);
     if (st != GMS_SUCCESS) return st;
-    // The matrix, padded to 256 bytes, then the partials of its scan.  Every
-    // matrix word and partial is written before it is read, so nothing is
-    // cleared between calls.
+    // The scan statuses, then the matrix.  Every matrix word is written
+    // before it is read, and the prescan clears the statuses the scan reads,
+    // so nothing is cleared between calls.
     const bool kv = vin != nullptr;
     const int lb = ceil_log2_u32(num_buckets);
     const uint32_t tile = tile_keys(lb, kv, n);
     const size_t m = ((size_t)1 << lb) * (((size_t)n + tile - 1) / tile);
-    const size_t m_pad = (m + 63) / 64 * 64;
-    if (m_pad > kNoAllocWords || scan_partials_needed(m) > kNoAllocWords - m_pad)
-        return GMS_ERR_CAPACITY;
+    if (m > kNoAllocWords - kNoAllocHead) return GMS_ERR_CAPACITY;
     st = check_buffers(kin, vin, kout, vout, n);
     if (st != GMS_SUCCESS) return st;
     uint32_t* base = nullptr;
     st = noalloc_scratch(&base);
     if (st != GMS_SUCCESS) return st;
     Scratch s;
-    s.hist = base;
-    s.hist_cap = m;
-    s.part = base + m_pad;
-    s.part_cap = kNoAllocWords - m_pad;
+    s.status = reinterpret_cast<uint64_t*>(base);
+    s.status_cap = kNoAllocStatus;
+    s.hist = base + kNoAllocHead;
+    s.hist_cap = kNoAllocWords - kNoAllocHead;
     return run_multisplit(s, kin, vin, kout, vout, n, num_buckets, d, d_offsets,
                           (hipStream_t)stream, nullptr);
 }

@@ -215,6 +215,37 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
     return x;
 }
 
+// Device-scope atomic load / store of a 64-bit status word and a 32-bit
+// counter: coherent across the XCDs' L2 caches (the value travels in the
+// word itself, so no ordering beyond the word is needed).
+__device__ __forceinline__ uint64_t load_device_u64(const uint64_t* p) {
+#ifdef GMS_HOST_EMULATION
+    return __atomic_load_n(p, __ATOMIC_RELAXED);
+#else
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+__device__ __forceinline__ void store_device_u64(uint64_t* p, uint64_t v) {
+#ifdef GMS_HOST_EMULATION
+    __atomic_store_n(p, v, __ATOMIC_RELAXED);
+#else
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+__device__ __forceinline__ void store_device_u32(uint32_t* p, uint32_t v) {
+#ifdef GMS_HOST_EMULATION
+    __atomic_store_n(p, v, __ATOMIC_RELAXED);
+#else
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+// Short pause in a wait loop.
+__device__ __forceinline__ void wait_pause() {
+#ifndef GMS_HOST_EMULATION
+    __builtin_amdgcn_s_sleep(2);
+#endif
+}
+
 // Value of lane 63 broadcast to the wave (the total after an inclusive scan).
 __device__ __forceinline__ uint32_t wave_last(uint32_t x) {
     return (uint32_t)GMS_READLANE((int)x, 63);

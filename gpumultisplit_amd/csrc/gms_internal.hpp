@@ -25,6 +25,10 @@ struct MsCall {
     uint32_t* H;  // histogram matrix, num_buckets(pow2 of log_b) * num_tiles
     int vec_ok;
     hipStream_t stream;
+    // Status words of the single-pass scan that follows, cleared by the
+    // prescan (NULL: the scan that follows is the two-launch one).
+    uint64_t* scan_status = nullptr;
+    uint32_t scan_chunks = 0;
 };
 
 // Functor-family launchers (one translation unit per family).
@@ -43,6 +47,12 @@ hipError_t launch_postscan(const MsCall& c, const DigitOfF& f);
 hipError_t launch_exclusive_scan(const uint32_t* in, uint32_t* out, uint32_t m, uint32_t* part,
                                  hipStream_t stream);
 size_t scan_partials_needed(size_t m);
+// The same scan in one launch.  `status` holds scan_chunks(m) 64-bit words
+// that must be zero when it starts (the plain path's prescan clears them);
+// scans sharing them must not overlap in time.
+uint32_t scan_chunks(size_t m);
+hipError_t launch_scan_single(const uint32_t* in, uint32_t* out, uint32_t m, uint64_t* status,
+                              hipStream_t stream);
 hipError_t launch_gather_offsets(const uint32_t* G, uint32_t num_tiles, uint32_t num_buckets,
                                  uint32_t* out, hipStream_t stream);
 hipError_t launch_fill(uint32_t* p, uint32_t v, uint32_t n, hipStream_t stream);
@@ -67,6 +77,8 @@ struct Scratch {
     uint32_t* part = nullptr;
     size_t part_cap = 0;
     uint32_t* offs = nullptr;  // 256 bucket starts (staging for host offsets)
+    uint64_t* status = nullptr;  // single-pass scan statuses of the plain path
+    size_t status_cap = 0;       // 64-bit words
     int device = -1;
 };
 

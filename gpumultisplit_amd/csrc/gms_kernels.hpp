@@ -178,10 +178,16 @@ __device__ __forceinline__ void prescan_tile(const uint32_t (&x)[K], uint32_t nv
 // t.  Tiles are handed to workgroups in the XCD order (xcd_tile), so that
 // neighbouring tiles' columns of H share L2 lines.  STREAM: streamed key
 // loads (long-tile inputs, larger than the caches).
+// `clear` (nclear 64-bit words, or NULL): the status words of the
+// single-pass scan that follows, zeroed by workgroup 0.
 template <int LOG_B, int K, bool VEC, bool STREAM, class F>
 __global__ __launch_bounds__(kThreads) void ms_prescan_kernel(const uint32_t* __restrict__ kin,
                                                               uint32_t n, uint32_t num_tiles,
-                                                              uint32_t* __restrict__ H, F f) {
+                                                              uint32_t* __restrict__ H, F f,
+                                                              uint64_t* __restrict__ clear,
+                                                              uint32_t nclear) {
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < nclear; i += kThreads) store_device_u64(&clear[i], 0ull);
     constexpr int B = 1 << LOG_B;
     constexpr uint32_t TILE = kThreads * K;
     __shared__ __attribute__((aligned(16))) uint32_t s_h[kWaves * B];

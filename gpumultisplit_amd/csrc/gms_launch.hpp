@@ -11,7 +11,8 @@ template <int LOG_B, bool KV, bool VEC, bool LARGE, class F>
 hipError_t launch_prescan_v(const MsCall& c, const F& f) {
     constexpr int K = tune_items(LOG_B, KV, LARGE);
     hipLaunchKernelGGL((ms_prescan_kernel<LOG_B, K, VEC, LARGE, F>), dim3(c.num_tiles),
-                       dim3(kThreads), 0, c.stream, c.kin, c.n, c.num_tiles, c.H, f);
+                       dim3(kThreads), 0, c.stream, c.kin, c.n, c.num_tiles, c.H, f,
+                       c.scan_status, c.scan_chunks);
     return hipGetLastError();
 }
 

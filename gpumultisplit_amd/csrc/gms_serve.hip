@@ -351,10 +351,10 @@ struct gms_server {
             return false;
         }
         for (auto& r : reqs) ++r->next;
+        st.step_items += reqs.size();
         inflight.push_back(Step{std::move(reqs), off, bytes, ev});
         slot ^= 1;
         ++st.steps;
-        st.step_items += reqs.size();
         st.keys_processed += keys;
         peak = arena.peak;
         return true;

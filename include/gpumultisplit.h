@@ -148,13 +148,12 @@ gms_status gms_multisplit(gms_multisplit_ctx *ctx, const uint32_t *d_key_in,
 /* No counterpart in the reference.  gms_multisplit without a context, for
  * callers that cannot allocate (code under hipGraph capture, serving loops,
  * harnesses that forbid hipMalloc): it allocates nothing, on the device or
- * the host.  Its scratch (the per-tile count matrix and its scan partials) is
+ * the host.  Its scratch (the per-tile count matrix and its scan state) is
  * a file-scope device array of GMS_NOALLOC_SCRATCH_BYTES in the library, one
  * copy on each GPU, held until the process ends.  Up to 1024 buckets; the
  * call needs 2^ceil(log2 B) * ceil(n / gms_tile_keys(B, key_value, n)) words
- * of matrix plus about 1/256 of that for the scan, and returns
- * GMS_ERR_CAPACITY beyond the array (2^27 key-value elements in 256 buckets
- * take 19 MiB).  Same results as gms_multisplit.  Every call on one GPU
+ * of matrix, and returns GMS_ERR_CAPACITY beyond the array less 16.5 KiB of
+ * scan state (2^27 key-value elements in 256 buckets take 19 MiB).  Same results as gms_multisplit.  Every call on one GPU
  * shares that GPU's array: calls that overlap in time on one GPU (two
  * streams, two host threads) race and give wrong results; calls on one
  * stream are ordered and safe. */

@@ -138,6 +138,10 @@ template <class T>
 inline hipError_t hipMalloc(T** p, size_t bytes) {
     return hipMalloc((void**)p, bytes);
 }
+inline hipError_t hipMemset(void* p, int v, size_t bytes) {
+    memset(p, v, bytes);
+    return hipSuccess;
+}
 inline hipError_t hipFree(void* p) {
     ++gms_emu::alloc_calls;
     free(p);

@@ -352,6 +352,12 @@ def test_emulated_server_mixed_requests(emu, max_inflight):
         assert st.completed == len(reqs) and st.submitted == len(reqs)
         assert st.active == 0 and st.pending == 0
         assert st.steps >= 4          # sorts take one step per digit
+        # every pass of every request went through one step
+        passes = sum(len(oracle.sort_schedule(r[0].digit_bits)) if r[0].op == 1 else 1
+                     for r in reqs)
+        assert st.step_items == passes
+        assert st.keys_processed == sum(
+            r[3] * (len(oracle.sort_schedule(r[0].digit_bits)) if r[0].op == 1 else 1) for r in reqs)
     finally:
         assert emu.gms_server_destroy(h) == 0
     _check_server_results(reqs)
