@@ -100,7 +100,7 @@ __device__ __forceinline__ uint32_t xcd_next(uint32_t* queue, uint32_t wgs, uint
     return (uint32_t)GMS_READFIRSTLANE((int)*s_next);
 }
 
-template <int LOG_B, int K, bool VEC>
+template <int LOG_B, int K, bool VEC, bool STREAM>
 __device__ __forceinline__ void batch_prescan_item(const BatchInst* __restrict__ inst,
                                                    const BatchWork* __restrict__ work, uint32_t w,
                                                    uint32_t* __restrict__ H, uint32_t* s_h) {
@@ -112,7 +112,7 @@ __device__ __forceinline__ void batch_prescan_item(const BatchInst* __restrict__
     uint32_t* Hi = H + I.mbase;
     uint32_t x[K];
     if (nvalid == TILE) {
-        load_striped4_full<K, VEC>(I.kin, base, x);
+        load_striped4_full<K, VEC, STREAM>(I.kin, base, x);
         prescan_tile<LOG_B, K, true>(x, TILE, it.tile, I.num_tiles, Hi, I.f, s_h);
     } else {
         load_striped4_guarded<K>(I.kin, base, nvalid, x);
@@ -120,19 +120,22 @@ __device__ __forceinline__ void batch_prescan_item(const BatchInst* __restrict__
     }
 }
 
-template <int LOG_B, int K, bool VEC, bool PERSIST>
+// STREAM: streamed key loads, for batches larger than the caches (whose
+// postscans cannot find the keys still cached).
+template <int LOG_B, int K, bool VEC, bool PERSIST, bool STREAM = false>
 __global__ __launch_bounds__(kThreads) void ms_batch_prescan_kernel(
     const BatchInst* __restrict__ inst, const BatchWork* __restrict__ work, uint32_t num_work,
     uint32_t* __restrict__ H, uint32_t* __restrict__ queue) {
     constexpr int B = 1 << LOG_B;
     __shared__ __attribute__((aligned(16))) uint32_t s_h[kWaves * B];
     if constexpr (!PERSIST) {
-        batch_prescan_item<LOG_B, K, VEC>(inst, work, xcd_tile(blockIdx.x, num_work), H, s_h);
+        batch_prescan_item<LOG_B, K, VEC, STREAM>(inst, work, xcd_tile(blockIdx.x, num_work), H,
+                                                  s_h);
     } else {
         __shared__ uint32_t s_next;
         const XcdRange R = xcd_range(num_work);
         for (uint32_t i = blockIdx.x / 8u; i < R.count; i = xcd_next(queue, R.wgs, &s_next))
-            batch_prescan_item<LOG_B, K, VEC>(inst, work, R.begin + i, H, s_h);
+            batch_prescan_item<LOG_B, K, VEC, STREAM>(inst, work, R.begin + i, H, s_h);
     }
 }
 
@@ -237,6 +240,7 @@ struct BatchLaunch {
     bool vec;               // every instance of the group 16-byte aligned
     bool persistent;
     hipStream_t stream;
+    bool stream_loads = false;  // the step is larger than the caches: prescan loads stream
 };
 
 hipError_t launch_batch_prescan_k8(const BatchLaunch& c);
@@ -255,16 +259,19 @@ int batch_occupancy_k32(int log_b, bool kv, bool postscan);
 
 template <int K>
 struct BatchKernels {
-    template <int LOG_B, bool VEC, bool P>
+    template <int LOG_B, bool VEC, bool P, bool S = false>
     static hipError_t prescan_v(const BatchLaunch& c) {
-        hipLaunchKernelGGL((ms_batch_prescan_kernel<LOG_B, K, VEC, P>), dim3(c.grid),
+        hipLaunchKernelGGL((ms_batch_prescan_kernel<LOG_B, K, VEC, P, S>), dim3(c.grid),
                            dim3(kThreads), 0, c.stream, c.inst, c.work, c.num_work, c.H, c.queue);
         return hipGetLastError();
     }
     template <int LOG_B>
     static hipError_t prescan(const BatchLaunch& c) {
-        if (c.persistent)
+        if (c.persistent) {
+            if (c.vec && c.stream_loads) return prescan_v<LOG_B, true, true, true>(c);
             return c.vec ? prescan_v<LOG_B, true, true>(c) : prescan_v<LOG_B, false, true>(c);
+        }
+        if (c.vec && c.stream_loads) return prescan_v<LOG_B, true, false, true>(c);
         return c.vec ? prescan_v<LOG_B, true, false>(c) : prescan_v<LOG_B, false, false>(c);
     }
     template <int LOG_B, bool KV, bool VEC, bool P>
