@@ -16,14 +16,15 @@ constexpr int kWaves = kThreads / 64;
 // Keys per thread, by log2(bucket count), key-value-ness and input size
 // (measured on MI355X).  24 everywhere, except for inputs of at least
 // kLongTileMin keys where longer tiles -- longer per-bucket output runs --
-// measured faster: 64-bucket key-value (32), 256-bucket key-value (28: the
-// call at 2^27 pairs 0.564 ms against 0.571 at 24, 0.576 at 20, 0.586 at 32)
-// and key-only from 128 buckets (48).  Below that size the longer tiles leave
+// measured faster: 64- and 256-bucket key-value (28: at 2^27 pairs the
+// 256-bucket call takes 0.564 ms against 0.571 at 24, 0.576 at 20, 0.586 at
+// 32; the 64-bucket one 0.515 against 0.523 at 24 and 0.532 at 32) and
+// key-only from 128 buckets (48).  Below that size the longer tiles leave
 // too few tiles per CU.
 constexpr uint32_t kLongTileMin = 1u << 26;
 
 constexpr int tune_items(int log_b, bool kv, bool large) {
-    return !large ? 24 : (kv ? (log_b == 6 ? 32 : (log_b == 8 ? 28 : 24)) : (log_b >= 7 ? 48 : 24));
+    return !large ? 24 : (kv ? (log_b == 6 || log_b == 8 ? 28 : 24) : (log_b >= 7 ? 48 : 24));
 }
 
 // Widest digit ranked in one LDS pass (2^w x 256 u32 counters): 1..4 bits in
