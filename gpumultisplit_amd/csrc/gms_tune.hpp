@@ -18,13 +18,17 @@ constexpr int kWaves = kThreads / 64;
 // kLongTileMin keys where longer tiles -- longer per-bucket output runs --
 // measured faster: 64- and 256-bucket key-value (28: at 2^27 pairs the
 // 256-bucket call takes 0.564 ms against 0.571 at 24, 0.576 at 20, 0.586 at
-// 32; the 64-bucket one 0.515 against 0.523 at 24 and 0.532 at 32) and
-// key-only from 128 buckets (48).  Below that size the longer tiles leave
+// 32; the 64-bucket one 0.515 against 0.523 at 24 and 0.532 at 32), 512 and
+// 1024 buckets key-value (36, the longest tile that keeps two workgroups per
+// CU: 1024 buckets 0.820 ms against 0.939 at 24, 0.882 at 28, 0.901 at 32)
+// and key-only from 128 buckets (48).  Below that size the longer tiles leave
 // too few tiles per CU.
 constexpr uint32_t kLongTileMin = 1u << 26;
 
 constexpr int tune_items(int log_b, bool kv, bool large) {
-    return !large ? 24 : (kv ? (log_b == 6 || log_b == 8 ? 28 : 24) : (log_b >= 7 ? 48 : 24));
+    if (!large) return 24;
+    if (!kv) return log_b >= 7 ? 48 : 24;
+    return log_b >= 9 ? 36 : (log_b == 6 || log_b == 8 ? 28 : 24);
 }
 
 // Widest digit ranked in one LDS pass (2^w x 256 u32 counters): 1..4 bits in

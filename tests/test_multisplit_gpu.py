@@ -407,7 +407,7 @@ def test_one_pass_up_to_1024_buckets_properties(dev, n, dist, B, kv):
 
 @pytest.mark.slow
 @pytest.mark.parametrize("B,kv,extra", [(128, False, 12345), (64, True, 777), (256, False, 1),
-                                        (256, True, 6143)])
+                                        (256, True, 6143), (1024, True, 9215), (512, True, 9217)])
 def test_long_tiles_against_oracle(dev, B, kv, extra):
     """Inputs of 2^26 keys and more take the long-tile shape of some classes;
     ragged sizes there, bit-exact against the oracle."""
