@@ -21,13 +21,15 @@ constexpr int kWaves = kThreads / 64;
 // 32; the 64-bucket one 0.515 against 0.523 at 24 and 0.532 at 32), 512 and
 // 1024 buckets key-value (36, the longest tile that keeps two workgroups per
 // CU: 1024 buckets 0.820 ms against 0.939 at 24, 0.882 at 28, 0.901 at 32)
-// and key-only from 128 buckets (48).  Below that size the longer tiles leave
-// too few tiles per CU.
+// and key-only: 128 and 256 buckets 56 (256 buckets 0.379 ms against 0.388
+// at 48 and 0.394 at 40), 512 and 1024 buckets 48 (at 56 a 1024-bucket
+// workgroup no longer fits twice in a CU: 0.705 against 0.521 ms).  Below that
+// size the longer tiles leave too few tiles per CU.
 constexpr uint32_t kLongTileMin = 1u << 26;
 
 constexpr int tune_items(int log_b, bool kv, bool large) {
     if (!large) return 24;
-    if (!kv) return log_b >= 7 ? 48 : 24;
+    if (!kv) return log_b >= 9 ? 48 : (log_b >= 7 ? 56 : 24);
     return log_b >= 9 ? 36 : (log_b == 6 || log_b == 8 ? 28 : 24);
 }
 
