@@ -242,7 +242,7 @@ static gms_status plan_call(Scratch& s, const uint32_t* kin, const uint32_t* vin
     const uint32_t nt = (uint32_t)(((size_t)n + tile - 1) / tile);
     const size_t m = ((size_t)1 << lb) * nt;
     if (m > s.hist_cap || !scan_fits(s, m)) return GMS_ERR_CAPACITY;
-    c = MsCall{kin, vin, kout, vout, n, lb, kv, n >= kLongTileMin, nt, s.hist,
+    c = MsCall{kin, vin, kout, vout, n, lb, kv, n >= long_tile_min(lb, kv), nt, s.hist,
                vec_aligned(kin) && vec_aligned(vin), stream};
     if (single_scan(s, m)) {
         c.scan_status = s.status;

@@ -322,7 +322,7 @@ gms_status BatchEngine::launch(const BatchDevice& d, hipStream_t s) {
         BatchLaunch c{d.inst, d.work + g.w0, g.w1 - g.w0, grid_for(g, post), d.mat,
                       (uint32_t)m_total, (uint32_t)keys_total, d.queue + 16 * gi + (post ? 8 : 0),
                       g.log_b, g.kv, g.vec, tuning.persistent != 0, s,
-                      keys_total >= kLongTileMin};
+                      keys_total >= kStreamLoadMin};
         switch (tuning.items_per_thread) {
             case 8: return post ? launch_batch_postscan_k8(c) : launch_batch_prescan_k8(c);
             case 16: return post ? launch_batch_postscan_k16(c) : launch_batch_prescan_k16(c);

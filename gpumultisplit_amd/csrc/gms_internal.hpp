@@ -20,7 +20,7 @@ struct MsCall {
     uint32_t n;
     int log_b;
     bool kv;
-    bool large;  // n >= kLongTileMin: the long-tile shape of the class
+    bool large;  // n >= long_tile_min(log_b, kv): the long-tile shape of the class
     uint32_t num_tiles;
     uint32_t* H;  // histogram matrix, num_buckets(pow2 of log_b) * num_tiles
     int vec_ok;

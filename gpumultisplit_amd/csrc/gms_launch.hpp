@@ -6,11 +6,12 @@
 
 namespace gms {
 
-// Long-tile inputs (LARGE) stream their key loads in the prescan.
-template <int LOG_B, bool KV, bool VEC, bool LARGE, class F>
+// Inputs of at least kStreamLoadMin keys (always long-tile ones) stream
+// their key loads in the prescan (STREAM).
+template <int LOG_B, bool KV, bool VEC, bool LARGE, bool STREAM, class F>
 hipError_t launch_prescan_v(const MsCall& c, const F& f) {
     constexpr int K = tune_items(LOG_B, KV, LARGE);
-    hipLaunchKernelGGL((ms_prescan_kernel<LOG_B, K, VEC, LARGE, F>), dim3(c.num_tiles),
+    hipLaunchKernelGGL((ms_prescan_kernel<LOG_B, K, VEC, STREAM, F>), dim3(c.num_tiles),
                        dim3(kThreads), 0, c.stream, c.kin, c.n, c.num_tiles, c.H, f,
                        c.scan_status, c.scan_chunks);
     return hipGetLastError();
@@ -18,11 +19,18 @@ hipError_t launch_prescan_v(const MsCall& c, const F& f) {
 
 template <int LOG_B, bool KV, class F>
 hipError_t launch_prescan_t(const MsCall& c, const F& f) {
-    if (c.large)
-        return c.vec_ok ? launch_prescan_v<LOG_B, KV, true, true>(c, f)
-                        : launch_prescan_v<LOG_B, KV, false, true>(c, f);
-    return c.vec_ok ? launch_prescan_v<LOG_B, KV, true, false>(c, f)
-                    : launch_prescan_v<LOG_B, KV, false, false>(c, f);
+    static_assert(kStreamLoadMin >= kLongTileMin, "streamed inputs take the long tiles");
+    if (c.large) {
+        if constexpr (long_tile_min(LOG_B, KV) < kStreamLoadMin) {
+            if (c.n < kStreamLoadMin)
+                return c.vec_ok ? launch_prescan_v<LOG_B, KV, true, true, false>(c, f)
+                                : launch_prescan_v<LOG_B, KV, false, true, false>(c, f);
+        }
+        return c.vec_ok ? launch_prescan_v<LOG_B, KV, true, true, true>(c, f)
+                        : launch_prescan_v<LOG_B, KV, false, true, true>(c, f);
+    }
+    return c.vec_ok ? launch_prescan_v<LOG_B, KV, true, false, false>(c, f)
+                    : launch_prescan_v<LOG_B, KV, false, false, false>(c, f);
 }
 
 template <int LOG_B, bool KV, bool VEC, bool LARGE, class F>

@@ -24,8 +24,18 @@ constexpr int kWaves = kThreads / 64;
 // and key-only: 128 and 256 buckets 56 (256 buckets 0.379 ms against 0.388
 // at 48 and 0.394 at 40), 512 and 1024 buckets 48 (at 56 a 1024-bucket
 // workgroup no longer fits twice in a CU: 0.705 against 0.521 ms).  Below that
-// size the longer tiles leave too few tiles per CU.
+// size the longer tiles leave too few tiles per CU, except for 1024-bucket
+// key-value, whose long tiles pay from 2^24 keys (0.116 ms against 0.132 at
+// 2^24, 0.229 against 0.266 at 2^25; 512 buckets there are within 2 % either
+// way, 256 buckets at 2^25 0.161 against 0.156).
 constexpr uint32_t kLongTileMin = 1u << 26;
+constexpr uint32_t long_tile_min(int log_b, bool kv) {
+    return kv && log_b >= 10 ? kLongTileMin / 4 : kLongTileMin;
+}
+// The prescan streams its key loads past the caches from this many keys;
+// smaller inputs are re-read by the postscan while still in the last-level
+// cache.
+constexpr uint32_t kStreamLoadMin = 1u << 26;
 
 constexpr int tune_items(int log_b, bool kv, bool large) {
     if (!large) return 24;
@@ -67,7 +77,7 @@ constexpr uint32_t kSinglePassBuckets = 1024;
 // Keys per tile for an input of n keys.  The shortest tile, used to size the
 // histogram matrix for every n up to a capacity, is tile_keys(log_b, kv, 0).
 constexpr uint32_t tile_keys(int log_b, bool kv, uint32_t n) {
-    return (uint32_t)(kThreads * tune_items(log_b, kv, n >= kLongTileMin));
+    return (uint32_t)(kThreads * tune_items(log_b, kv, n >= long_tile_min(log_b, kv)));
 }
 
 inline int ceil_log2_u32(uint32_t x) {

@@ -80,3 +80,17 @@ def test_more_than_256_buckets_2e25(dev):
     np.testing.assert_array_equal(got[0], wk)
     np.testing.assert_array_equal(got[1], wv)
     np.testing.assert_array_equal(got[2], wo)
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("B,n", [(1024, (1 << 24) - 1), (1000, (1 << 24) + 9215), (1024, 1 << 24)])
+def test_1024_buckets_around_long_tile_switch(dev, B, n):
+    """1024-bucket key-value inputs switch to the long tiles at 2^24 keys: just
+    below, at, and past the switch with a ragged last tile, against the oracle."""
+    banner(f"B={B} n={n} tile={gms.tile_keys(B, True, n)}")
+    keys = W.uniform_keys(n, seed=B + n)
+    got = _run(dev, keys, B, gms.range_bucket(B), True)
+    wk, wv, wo = oracle.multisplit(keys, B, oracle.RANGE, 0, np.arange(n, dtype=np.uint32))
+    np.testing.assert_array_equal(got[0], wk)
+    np.testing.assert_array_equal(got[1], wv)
+    np.testing.assert_array_equal(got[2], wo)
