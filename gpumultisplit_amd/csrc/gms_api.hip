@@ -75,7 +75,7 @@ size_t matrix_elems(uint32_t n, uint32_t num_buckets) {
     const size_t t_ko = ((size_t)n + tko - 1) / tko;
     const size_t t_kv = ((size_t)n + tkv - 1) / tkv;
     // the shortest tile of the 64-bit kernels (gms_multisplit_wide, gms_sort_wide)
-    const size_t t_w = ((size_t)n + wide_tile_elems(2, 2) - 1) / wide_tile_elems(2, 2);
+    const size_t t_w = ((size_t)n + wide_min_tile_elems() - 1) / wide_min_tile_elems();
     // one column per wave subproblem of the wave-level variant (up to 64 buckets)
     const size_t t_wms = num_buckets <= kWmsMaxBuckets
                              ? ((size_t)n + wms_subproblem_keys() - 1) / wms_subproblem_keys()

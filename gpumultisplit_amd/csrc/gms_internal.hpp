@@ -148,6 +148,8 @@ int sort_schedule(uint32_t digit_bits, SortPass* p);
 // Elements with 64-bit keys and/or values (gms_wide.hip): key_words 1 or 2,
 // value_words 0, 1 or 2, not both below 2; up to 256 buckets.
 uint32_t wide_tile_elems(uint32_t key_words, uint32_t value_words);
+// The shortest of those tiles over every supported element shape.
+uint32_t wide_min_tile_elems();
 gms_status run_wide_multisplit(Scratch& s, const uint32_t* kin, const uint32_t* vin,
                                uint32_t* kout, uint32_t* vout, uint32_t n, uint32_t key_words,
                                uint32_t value_words, uint32_t num_buckets,

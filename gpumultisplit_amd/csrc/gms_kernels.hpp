@@ -318,6 +318,7 @@ __device__ __forceinline__ void rank_pass(const uint32_t (&key)[K], const uint32
 template <int K, bool KV>
 __device__ __forceinline__ void reload_blocked(uint32_t (&key)[K], uint32_t (&val)[K],
                                                const uint32_t* s_keys, const uint32_t* s_vals) {
+    static_assert(K % 4 == 0, "16-byte LDS reads of whole rows");
     const uint32_t off = threadIdx.x * (uint32_t)K;
 #pragma unroll
     for (int i = 0; i < K / 4; ++i) {

@@ -14,12 +14,22 @@
 // the area the ranking counters used.  The write-out reads the packed word of
 // each slot and fetches the element's words through its position, so a
 // 16-byte element costs the ranking no more LDS traffic than a 4-byte key.
+#include <algorithm>
+
 #include "gms_wide_kernels.hpp"
 
 namespace gms {
 
 uint32_t wide_tile_elems(uint32_t key_words, uint32_t value_words) {
     return kThreads * (uint32_t)wide_items((int)key_words, (int)value_words);
+}
+
+uint32_t wide_min_tile_elems() {
+    uint32_t m = wide_tile_elems(2, 0);
+    for (uint32_t kw = 1; kw <= 2; ++kw)
+        for (uint32_t vw = 1; vw <= 2; ++vw)
+            if (kw == 2 || vw == 2) m = std::min(m, wide_tile_elems(kw, vw));
+    return m;
 }
 
 template <class Mk64, class Mk32>

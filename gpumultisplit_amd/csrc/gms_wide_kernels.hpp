@@ -23,9 +23,12 @@
 namespace gms {
 
 // Elements per thread of a tile: TILE * (KW + VW) words of payload share LDS
-// with the packed words, so wider elements take shorter tiles (3 workgroups
-// per CU for 2 and 3 words per element, 2 for 4).
-constexpr int wide_items(int kw, int vw) { return kw + vw <= 2 ? 16 : 12; }
+// with the packed words, so wider elements take shorter tiles: 2 words per
+// element K = 20 (60 KB, 2 workgroups per CU; 256-bucket key-only 64-bit keys
+// at 2^27: 0.761 ms against 0.773 at K = 16 with 3 per CU, 0.760-0.766 at 24),
+// 3 words K = 12 (3 per CU), 4 words K = 12 (2 per CU; K = 16 would leave one).
+// K is a multiple of 4 (16-byte LDS row reads).
+constexpr int wide_items(int kw, int vw) { return kw + vw <= 2 ? 20 : 12; }
 
 // Packed word -> bucket id (the ranking's identifier).
 struct PackedBidF {
