@@ -48,11 +48,20 @@ static gms_status wide_stages(Scratch& s, const uint32_t* kin, const uint32_t* v
     const uint32_t tile = wide_tile_elems(kw, vw);
     const uint32_t nt = (uint32_t)(((size_t)n + tile - 1) / tile);
     const size_t m = ((size_t)1 << lb) * nt;
-    if (m > s.hist_cap || scan_partials_needed(m) > s.part_cap) return GMS_ERR_CAPACITY;
+    // One-launch scan when the scratch has its status words (the prescan
+    // clears them), else the two-launch one.
+    const bool one = s.status && scan_chunks(m) <= s.status_cap;
+    if (m > s.hist_cap || (!one && scan_partials_needed(m) > s.part_cap)) return GMS_ERR_CAPACITY;
     auto al16 = [](const void* p) { return p == nullptr || (((uintptr_t)p) & 15u) == 0; };
-    const WideCall c{kin, vin, kout, vout, n, nt, s.hist, al16(kin) && al16(vin), stream};
+    WideCall c{kin, vin, kout, vout, n, nt, s.hist, al16(kin) && al16(vin), stream};
+    if (one) {
+        c.scan_status = s.status;
+        c.scan_chunks = scan_chunks(m);
+    }
     hipError_t e = wide_dispatch(c, lb, kw, vw, mk64, mk32, true);
-    if (e == hipSuccess) e = launch_exclusive_scan(s.hist, s.hist, (uint32_t)m, s.part, stream);
+    if (e == hipSuccess)
+        e = one ? launch_scan_single(s.hist, s.hist, (uint32_t)m, s.status, stream)
+                : launch_exclusive_scan(s.hist, s.hist, (uint32_t)m, s.part, stream);
     if (e == hipSuccess) e = wide_dispatch(c, lb, kw, vw, mk64, mk32, false);
     if (e == hipSuccess && d_offsets) e = launch_gather_offsets(s.hist, nt, B, d_offsets, stream);
     return hip_status(e);

@@ -69,7 +69,9 @@ __device__ __forceinline__ void load_elems(const uint32_t* __restrict__ p, size_
 // 16-byte stripes (the 32-bit prescan's lane-contiguous loads); word
 // 4*(j*256+t)+i belongs to element (4*(j*256+t)+i) / KW, and a 64-bit key
 // never straddles a stripe.  Writes column t of the num_tiles-column matrix H.
-template <int LOG_B, int K, int KW, bool VEC, class F>
+// STREAM: streamed loads, for key arrays larger than the caches (as in the
+// 32-bit prescan).
+template <int LOG_B, int K, int KW, bool VEC, bool STREAM = false, class F>
 __device__ __forceinline__ void wide_prescan_tile(const uint32_t* __restrict__ kin, uint32_t n,
                                                   uint32_t t, uint32_t num_tiles,
                                                   uint32_t* __restrict__ H, F f, uint32_t* s_h) {
@@ -81,7 +83,7 @@ __device__ __forceinline__ void wide_prescan_tile(const uint32_t* __restrict__ k
     const uint32_t nvalid = tile_valid(n, t, TILE);
     const size_t base = (size_t)t * TILE * KW;
     uint32_t x[WPT];
-    if (nvalid == TILE) load_striped4_full<WPT, VEC>(kin, base, x);
+    if (nvalid == TILE) load_striped4_full<WPT, VEC, STREAM>(kin, base, x);
     else load_striped4_guarded<WPT>(kin, base, nvalid * KW, x);
     uint32_t* my = s_h + wid * B;
     for (uint32_t i = lane; i < (uint32_t)B; i += 64) my[i] = 0;
@@ -108,13 +110,19 @@ __device__ __forceinline__ void wide_prescan_tile(const uint32_t* __restrict__ k
     }
 }
 
-template <int LOG_B, int K, int KW, bool VEC, class F>
+// `clear` (nclear 64-bit words, or NULL): the status words of the
+// single-pass scan that follows, zeroed by workgroup 0.
+template <int LOG_B, int K, int KW, bool VEC, bool STREAM, class F>
 __global__ __launch_bounds__(kThreads) void ms_wide_prescan_kernel(const uint32_t* __restrict__ kin,
                                                                    uint32_t n, uint32_t num_tiles,
-                                                                   uint32_t* __restrict__ H, F f) {
+                                                                   uint32_t* __restrict__ H, F f,
+                                                                   uint64_t* __restrict__ clear,
+                                                                   uint32_t nclear) {
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < nclear; i += kThreads) store_device_u64(&clear[i], 0ull);
     __shared__ __attribute__((aligned(16))) uint32_t s_h[kWaves * (1 << LOG_B)];
-    wide_prescan_tile<LOG_B, K, KW, VEC>(kin, n, xcd_tile(blockIdx.x, num_tiles), num_tiles, H, f,
-                                         s_h);
+    wide_prescan_tile<LOG_B, K, KW, VEC, STREAM>(kin, n, xcd_tile(blockIdx.x, num_tiles),
+                                                 num_tiles, H, f, s_h);
 }
 
 template <int LOG_B, int K, int LEVEL_BITS, int KW, int VW, bool FULL, class F>
@@ -244,13 +252,27 @@ struct WideCall {
     uint32_t* H;
     bool vec;
     hipStream_t stream;
+    // Status words of the single-pass scan that follows, cleared by the
+    // prescan (NULL: the scan that follows is the two-launch one).
+    uint64_t* scan_status = nullptr;
+    uint32_t scan_chunks = 0;
 };
 
 template <int LOG_B, int KW, int VW, bool VEC, class F>
 inline hipError_t wide_prescan_v(const WideCall& c, const F& f) {
     constexpr int K = wide_items(KW, VW);
-    hipLaunchKernelGGL((ms_wide_prescan_kernel<LOG_B, K, KW, VEC, F>), dim3(c.num_tiles),
-                       dim3(kThreads), 0, c.stream, c.kin, c.n, c.num_tiles, c.H, f);
+    // Key arrays of at least kStreamLoadMin words stream their loads.
+    if constexpr (VEC) {
+        if ((size_t)c.n * KW >= kStreamLoadMin) {
+            hipLaunchKernelGGL((ms_wide_prescan_kernel<LOG_B, K, KW, true, true, F>),
+                               dim3(c.num_tiles), dim3(kThreads), 0, c.stream, c.kin, c.n,
+                               c.num_tiles, c.H, f, c.scan_status, c.scan_chunks);
+            return hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL((ms_wide_prescan_kernel<LOG_B, K, KW, VEC, false, F>), dim3(c.num_tiles),
+                       dim3(kThreads), 0, c.stream, c.kin, c.n, c.num_tiles, c.H, f,
+                       c.scan_status, c.scan_chunks);
     return hipGetLastError();
 }
 
