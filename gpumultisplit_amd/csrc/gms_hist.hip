@@ -33,7 +33,7 @@ constexpr int kRangeCopies = 4;
 // dominant bin) then add to C addresses instead of one, which the LDS
 // serialises per address; evenly spread keys see the same bank spread as
 // with one copy.
-template <bool VEC, int W = 1, int C = kHistCopies, class BinsOf>
+template <bool VEC, int W = 1, int C = kHistCopies, bool STREAM = false, class BinsOf>
 __device__ __forceinline__ void hist_body(const uint32_t* __restrict__ p, uint32_t n, uint32_t nbins,
                                           uint32_t* __restrict__ d_hist, BinsOf bins_of) {
     static_assert(W == 1 || W == 2, "elements of 1 or 2 words");
@@ -48,11 +48,11 @@ __device__ __forceinline__ void hist_body(const uint32_t* __restrict__ p, uint32
     const uint32_t nfull = (uint32_t)((size_t)n * W / kHistTile);
     uint32_t t = blockIdx.x;
     uint32_t x[kHistItems];
-    if (t < nfull) load_striped4_full<kHistItems, VEC>(p, (size_t)t * kHistTile, x);
+    if (t < nfull) load_striped4_full<kHistItems, VEC, STREAM>(p, (size_t)t * kHistTile, x);
     while (t < nfull) {
         const uint32_t tn = t + gridDim.x;
         uint32_t y[kHistItems];
-        if (tn < nfull) load_striped4_full<kHistItems, VEC>(p, (size_t)tn * kHistTile, y);
+        if (tn < nfull) load_striped4_full<kHistItems, VEC, STREAM>(p, (size_t)tn * kHistTile, y);
         uint32_t b[kHistItems];
         bins_of(x, b);
 #pragma unroll
@@ -92,19 +92,22 @@ __device__ __forceinline__ auto per_word(F f) {
     };
 }
 
-template <bool VEC, class F>
+// STREAM: streamed loads, for inputs of at least kStreamLoadMin words (each
+// line is read once and the input is larger than the caches).
+template <bool VEC, bool STREAM, class F>
 __global__ __launch_bounds__(kThreads) void hist_u32_kernel(const uint32_t* __restrict__ kin,
                                                             uint32_t n, uint32_t nbins,
                                                             uint32_t* __restrict__ d_hist, F f) {
-    hist_body<VEC>(kin, n, nbins, d_hist, per_word([&](uint32_t k) { return f(k); }));
+    hist_body<VEC, 1, kHistCopies, STREAM>(kin, n, nbins, d_hist,
+                                           per_word([&](uint32_t k) { return f(k); }));
 }
 
-template <bool VEC>
+template <bool VEC, bool STREAM>
 __global__ __launch_bounds__(kThreads) void hist_even_f32_kernel(const uint32_t* __restrict__ xbits,
                                                                  uint32_t n, uint32_t nbins,
                                                                  uint32_t* __restrict__ d_hist,
                                                                  float lower, float inv_delta) {
-    hist_body<VEC>(xbits, n, nbins, d_hist, per_word([&](uint32_t w) {
+    hist_body<VEC, 1, kHistCopies, STREAM>(xbits, n, nbins, d_hist, per_word([&](uint32_t w) {
         const float d = __uint_as_float(w) - lower;
         // The product is rounded on its own (no contraction with a later add);
         // negative, NaN and >= nbins samples map to an out-of-range bin.
@@ -244,11 +247,11 @@ __global__ __launch_bounds__(kThreads) void hist_range_kernel(const uint32_t* __
 
 // 64-bit keys through a 64-bit identifier: the same prefetching resident
 // grid over the key array viewed as words, each (low, high) word pair one key.
-template <bool VEC>
+template <bool VEC, bool STREAM>
 __global__ __launch_bounds__(kThreads) void hist_u64_kernel(const uint32_t* __restrict__ words,
                                                             uint32_t n, uint32_t nbins,
                                                             uint32_t* __restrict__ d_hist, AnyF64 f) {
-    hist_body<VEC, 2>(words, n, nbins, d_hist, [&](auto& x, auto& b) {
+    hist_body<VEC, 2, kHistCopies, STREAM>(words, n, nbins, d_hist, [&](auto& x, auto& b) {
         constexpr int N = (int)(sizeof(x) / sizeof(x[0]));
 #pragma unroll
         for (int k = 0; k < N; k += 2) {
@@ -264,30 +267,37 @@ static uint32_t hist_grid(uint32_t n, uint32_t max_grid = kHistGrid) {
 }
 
 static bool aligned16(const void* p) { return (((uintptr_t)p) & 15u) == 0; }
+static bool stream_words(size_t words) { return words >= kStreamLoadMin; }
 
 hipError_t launch_histogram_u64(const uint64_t* keys, uint32_t n, uint32_t* d_hist,
                                 uint32_t nbins, const AnyF64& f, hipStream_t stream) {
     const size_t full = (size_t)n * 2 / kHistTile;
     const uint32_t grid = full < kHistGrid ? (full ? (uint32_t)full : 1u) : kHistGrid;
     const uint32_t* w = reinterpret_cast<const uint32_t*>(keys);
-    if (aligned16(keys))
-        hipLaunchKernelGGL((hist_u64_kernel<true>), dim3(grid), dim3(kThreads), 0, stream, w, n,
-                           nbins, d_hist, f);
+    if (aligned16(keys) && stream_words((size_t)n * 2))
+        hipLaunchKernelGGL((hist_u64_kernel<true, true>), dim3(grid), dim3(kThreads), 0, stream,
+                           w, n, nbins, d_hist, f);
+    else if (aligned16(keys))
+        hipLaunchKernelGGL((hist_u64_kernel<true, false>), dim3(grid), dim3(kThreads), 0, stream,
+                           w, n, nbins, d_hist, f);
     else
-        hipLaunchKernelGGL((hist_u64_kernel<false>), dim3(grid), dim3(kThreads), 0, stream, w, n,
-                           nbins, d_hist, f);
+        hipLaunchKernelGGL((hist_u64_kernel<false, false>), dim3(grid), dim3(kThreads), 0, stream,
+                           w, n, nbins, d_hist, f);
     return hipGetLastError();
 }
 
 template <class F>
 static hipError_t launch_hist(const uint32_t* kin, uint32_t n, uint32_t* d_hist, uint32_t nbins,
                               const F& f, hipStream_t stream) {
-    if (aligned16(kin))
-        hipLaunchKernelGGL((hist_u32_kernel<true, F>), dim3(hist_grid(n)), dim3(kThreads), 0,
-                           stream, kin, n, nbins, d_hist, f);
+    if (aligned16(kin) && stream_words(n))
+        hipLaunchKernelGGL((hist_u32_kernel<true, true, F>), dim3(hist_grid(n)), dim3(kThreads),
+                           0, stream, kin, n, nbins, d_hist, f);
+    else if (aligned16(kin))
+        hipLaunchKernelGGL((hist_u32_kernel<true, false, F>), dim3(hist_grid(n)), dim3(kThreads),
+                           0, stream, kin, n, nbins, d_hist, f);
     else
-        hipLaunchKernelGGL((hist_u32_kernel<false, F>), dim3(hist_grid(n)), dim3(kThreads), 0,
-                           stream, kin, n, nbins, d_hist, f);
+        hipLaunchKernelGGL((hist_u32_kernel<false, false, F>), dim3(hist_grid(n)), dim3(kThreads),
+                           0, stream, kin, n, nbins, d_hist, f);
     return hipGetLastError();
 }
 
@@ -337,12 +347,15 @@ hipError_t launch_histogram_even_f32(const float* x, uint32_t n, int32_t* d_hist
                                      float lower, float inv_delta, hipStream_t stream) {
     const uint32_t* xb = reinterpret_cast<const uint32_t*>(x);
     uint32_t* h = reinterpret_cast<uint32_t*>(d_hist);
-    if (aligned16(x))
-        hipLaunchKernelGGL((hist_even_f32_kernel<true>), dim3(hist_grid(n)), dim3(kThreads), 0,
-                           stream, xb, n, nbins, h, lower, inv_delta);
+    if (aligned16(x) && stream_words(n))
+        hipLaunchKernelGGL((hist_even_f32_kernel<true, true>), dim3(hist_grid(n)), dim3(kThreads),
+                           0, stream, xb, n, nbins, h, lower, inv_delta);
+    else if (aligned16(x))
+        hipLaunchKernelGGL((hist_even_f32_kernel<true, false>), dim3(hist_grid(n)),
+                           dim3(kThreads), 0, stream, xb, n, nbins, h, lower, inv_delta);
     else
-        hipLaunchKernelGGL((hist_even_f32_kernel<false>), dim3(hist_grid(n)), dim3(kThreads), 0,
-                           stream, xb, n, nbins, h, lower, inv_delta);
+        hipLaunchKernelGGL((hist_even_f32_kernel<false, false>), dim3(hist_grid(n)),
+                           dim3(kThreads), 0, stream, xb, n, nbins, h, lower, inv_delta);
     return hipGetLastError();
 }
 
