@@ -205,13 +205,14 @@ gms_status gms_multisplit_set_variant(gms_multisplit_ctx *ctx, int variant);
 uint32_t gms_subproblem_size_wms(uint32_t num_buckets, int key_value, uint32_t *size_block);
 
 /* subproblem_size_bms_key_only / _key_value (src/api/bms_api.h:23-89): keys
- * per tile for a bucket count (the tile of inputs below 2^26 keys, also the
- * shortest tile of any input).  Returns 0 for an unsupported count. */
+ * per tile for a bucket count (the tile of short inputs, also the shortest
+ * tile of any input).  Returns 0 for an unsupported count. */
 uint32_t gms_subproblem_size(uint32_t num_buckets, int key_value);
 
-/* Keys per tile of an n-key call (inputs of 2^26 keys and more may use a
- * longer tile); no counterpart in the reference, whose tile depends on the
- * bucket count only.  Returns 0 for an unsupported count. */
+/* Keys per tile of an n-key call (inputs of 2^26 keys and more, 2^24 for
+ * 1024-bucket key-value, may use a longer tile); no counterpart in the
+ * reference, whose tile depends on the bucket count only.  Returns 0 for an
+ * unsupported count. */
 uint32_t gms_tile_keys(uint32_t num_buckets, int key_value, uint32_t n);
 
 /* Stage timing for measurement: when enabled, every multisplit on this context
