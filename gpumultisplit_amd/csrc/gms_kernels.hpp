@@ -203,6 +203,66 @@ __global__ __launch_bounds__(kThreads) void ms_prescan_kernel(const uint32_t* __
     }
 }
 
+// Counts of one tile into s_sum[b * TPW + j], through the per-wave rows of s_h.
+template <int LOG_B, int K, int TPW, bool FULL, class F>
+__device__ __forceinline__ void group_count(const uint32_t (&x)[K], uint32_t nvalid, F f,
+                                            uint32_t* s_h, uint32_t* s_sum, uint32_t j) {
+    constexpr int B = 1 << LOG_B;
+    const uint32_t tid = threadIdx.x;
+    if constexpr (LOG_B > 3) {
+        uint32_t* my = s_h + (tid >> 6) * B;
+        for (uint32_t i = tid & 63u; i < (uint32_t)B; i += 64) my[i] = 0;
+        GMS_WAVE_SYNC();
+    }
+    prescan_count<LOG_B, K, FULL>(x, nvalid, f, s_h);
+    __syncthreads();
+    for (uint32_t b = tid; b < (uint32_t)B; b += kThreads) {
+        uint32_t s = 0;
+#pragma unroll
+        for (int q = 0; q < kWaves; ++q) s += s_h[q * B + b];
+        s_sum[b * TPW + j] = s;
+    }
+    __syncthreads();
+}
+
+// Prescan of TPW adjacent tiles per workgroup (groups in the XCD order),
+// counted one after the other; the counts of one bucket in the group's tiles
+// are TPW adjacent words of H, written by TPW adjacent lanes of one store.
+// Pays where the matrix is large against the keys (1024 buckets: 9216-key
+// tiles write 1024 words each): prescan 122 against 137 us at 2^27 key-value
+// pairs on MI355X; at 64 buckets it loses (86 against 79 us).
+template <int LOG_B, int K, bool VEC, bool STREAM, int TPW, class F>
+__global__ __launch_bounds__(kThreads) void ms_prescan_group_kernel(
+    const uint32_t* __restrict__ kin, uint32_t n, uint32_t num_tiles, uint32_t* __restrict__ H,
+    F f, uint64_t* __restrict__ clear, uint32_t nclear) {
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < nclear; i += kThreads) store_device_u64(&clear[i], 0ull);
+    constexpr int B = 1 << LOG_B;
+    constexpr uint32_t TILE = kThreads * K;
+    __shared__ __attribute__((aligned(16))) uint32_t s_h[kWaves * B];
+    __shared__ uint32_t s_sum[B * TPW];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t ngroups = (num_tiles + TPW - 1) / TPW;
+    const uint32_t t0 = xcd_tile(blockIdx.x, ngroups) * (uint32_t)TPW;
+    const uint32_t nt = num_tiles - t0 < (uint32_t)TPW ? num_tiles - t0 : (uint32_t)TPW;
+    for (uint32_t j = 0; j < nt; ++j) {
+        uint32_t x[K];
+        const uint32_t t = t0 + j;
+        const uint32_t nv = tile_valid(n, t, TILE);
+        if (nv == TILE) {
+            load_striped4_full<K, VEC, STREAM>(kin, (size_t)t * TILE, x);
+            group_count<LOG_B, K, TPW, true>(x, TILE, f, s_h, s_sum, j);
+        } else {
+            load_striped4_guarded<K>(kin, (size_t)t * TILE, nv, x);
+            group_count<LOG_B, K, TPW, false>(x, nv, f, s_h, s_sum, j);
+        }
+    }
+    for (uint32_t i = tid; i < (uint32_t)(B * TPW); i += kThreads) {
+        const uint32_t b = i / TPW, j = i % TPW;
+        if (j < nt) H[(size_t)b * num_tiles + t0 + j] = s_sum[i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // One stable LDS ranking pass over the digit (bucket >> SHIFT) & (2^W - 1).
 // On entry key/val hold the thread's K consecutive keys of the current order

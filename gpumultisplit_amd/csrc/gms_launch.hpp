@@ -11,9 +11,17 @@ namespace gms {
 template <int LOG_B, bool KV, bool VEC, bool LARGE, bool STREAM, class F>
 hipError_t launch_prescan_v(const MsCall& c, const F& f) {
     constexpr int K = tune_items(LOG_B, KV, LARGE);
-    hipLaunchKernelGGL((ms_prescan_kernel<LOG_B, K, VEC, STREAM, F>), dim3(c.num_tiles),
-                       dim3(kThreads), 0, c.stream, c.kin, c.n, c.num_tiles, c.H, f,
-                       c.scan_status, c.scan_chunks);
+    if constexpr (LARGE && KV && LOG_B >= 10) {
+        // groups of 4 tiles (measured for 1024-bucket key-value only)
+        constexpr int TPW = 4;
+        hipLaunchKernelGGL((ms_prescan_group_kernel<LOG_B, K, VEC, STREAM, TPW, F>),
+                           dim3((c.num_tiles + TPW - 1) / TPW), dim3(kThreads), 0, c.stream,
+                           c.kin, c.n, c.num_tiles, c.H, f, c.scan_status, c.scan_chunks);
+    } else {
+        hipLaunchKernelGGL((ms_prescan_kernel<LOG_B, K, VEC, STREAM, F>), dim3(c.num_tiles),
+                           dim3(kThreads), 0, c.stream, c.kin, c.n, c.num_tiles, c.H, f,
+                           c.scan_status, c.scan_chunks);
+    }
     return hipGetLastError();
 }
 
