@@ -11,12 +11,19 @@ namespace gms {
 template <int LOG_B, bool KV, bool VEC, bool LARGE, bool STREAM, class F>
 hipError_t launch_prescan_v(const MsCall& c, const F& f) {
     constexpr int K = tune_items(LOG_B, KV, LARGE);
-    if constexpr (LARGE && KV && LOG_B >= 10) {
-        // groups of 4 tiles (measured for 1024-bucket key-value only)
-        constexpr int TPW = 4;
-        hipLaunchKernelGGL((ms_prescan_group_kernel<LOG_B, K, VEC, STREAM, TPW, F>),
-                           dim3((c.num_tiles + TPW - 1) / TPW), dim3(kThreads), 0, c.stream,
-                           c.kin, c.n, c.num_tiles, c.H, f, c.scan_status, c.scan_chunks);
+    if constexpr (LARGE && KV && LOG_B >= 9) {
+        // Groups of 4 tiles at 512 buckets; at 1024, groups of 8 from 8192 tiles
+        // on (fewer groups than CUs below that: 2^24 keys 24 us against 17.5 in
+        // groups of 4).  Measured for key-value only.
+        if (LOG_B >= 10 && c.num_tiles >= 8192u) {
+            hipLaunchKernelGGL((ms_prescan_group_kernel<LOG_B, K, VEC, STREAM, 8, F>),
+                               dim3((c.num_tiles + 7) / 8), dim3(kThreads), 0, c.stream, c.kin,
+                               c.n, c.num_tiles, c.H, f, c.scan_status, c.scan_chunks);
+        } else {
+            hipLaunchKernelGGL((ms_prescan_group_kernel<LOG_B, K, VEC, STREAM, 4, F>),
+                               dim3((c.num_tiles + 3) / 4), dim3(kThreads), 0, c.stream, c.kin,
+                               c.n, c.num_tiles, c.H, f, c.scan_status, c.scan_chunks);
+        }
     } else {
         hipLaunchKernelGGL((ms_prescan_kernel<LOG_B, K, VEC, STREAM, F>), dim3(c.num_tiles),
                            dim3(kThreads), 0, c.stream, c.kin, c.n, c.num_tiles, c.H, f,
