@@ -231,9 +231,9 @@ __device__ __forceinline__ void group_count(const uint32_t (&x)[K], uint32_t nva
 // Pays where the matrix is large against the keys (512 and 1024 buckets:
 // 9216-key tiles write 512 or 1024 words each).  At 2^27 key-value pairs on
 // MI355X: 1024 buckets 116 us in groups of 8 tiles, 123 in groups of 4, 137
-// one tile per workgroup; 512 buckets 102 in groups of 4 against 112; key-only
-// 1024 buckets 106 against 120, 512 buckets 95 against 102.  At 64 buckets it
-// loses (86 against 79 us).
+// one tile per workgroup; 512 buckets 96 in groups of 8, 101 in groups of 4,
+// 112 one tile per workgroup; key-only 1024 buckets 106 against 120, 512
+// buckets 90 against 102.  At 64 buckets it loses (86 against 79 us).
 template <int LOG_B, int K, bool VEC, bool STREAM, int TPW, class F>
 __global__ __launch_bounds__(kThreads) void ms_prescan_group_kernel(
     const uint32_t* __restrict__ kin, uint32_t n, uint32_t num_tiles, uint32_t* __restrict__ H,

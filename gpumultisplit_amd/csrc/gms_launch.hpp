@@ -12,10 +12,10 @@ template <int LOG_B, bool KV, bool VEC, bool LARGE, bool STREAM, class F>
 hipError_t launch_prescan_v(const MsCall& c, const F& f) {
     constexpr int K = tune_items(LOG_B, KV, LARGE);
     if constexpr (LARGE && LOG_B >= 9) {
-        // Groups of 4 tiles at 512 buckets; at 1024, groups of 8 from 8192 tiles
-        // on (fewer groups than CUs below that: 2^24 key-value pairs 24 us
-        // against 17.5 in groups of 4).
-        if (LOG_B >= 10 && c.num_tiles >= 8192u) {
+        // Groups of 8 tiles from 8192 tiles on, else of 4 (8 would leave fewer
+        // groups than CUs: 1024 buckets at 2^24 key-value pairs 24 us against
+        // 17.5 in groups of 4).
+        if (c.num_tiles >= 8192u) {
             hipLaunchKernelGGL((ms_prescan_group_kernel<LOG_B, K, VEC, STREAM, 8, F>),
                                dim3((c.num_tiles + 7) / 8), dim3(kThreads), 0, c.stream, c.kin,
                                c.n, c.num_tiles, c.H, f, c.scan_status, c.scan_chunks);
